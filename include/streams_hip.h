@@ -1,0 +1,213 @@
+/*
+ * streams_hip.h — C-ABI of libstreams_hip.so, the gfx950 (MI355X) implementation of the
+ * STREAmS-2 multideal reacting RHS and stiff chemistry.
+ *
+ * Every entry point replaces one routine of the reference's CUDA-Fortran kernel module
+ * `streams_kernels_gpu` (src/multideal/kernels_gpu.F90) or of the halo layer
+ * (src/base_gpu.F90), and keeps its argument order.  Conventions:
+ *   - arrays are device pointers with the reference's Fortran layout (column-major,
+ *     first index fastest, lower bound 1-ng on padded fields: w(1-ng:nx+ng,1-ng:ny+ng,
+ *     1-ng:nz+ng,1:nv)); tables keep their Fortran shapes (e.g. cp_coeff(indx_cp_l:
+ *     indx_cp_r+2,N_S,nsetcv));
+ *   - N_S is a compile-time macro in the reference (src/multideal/index_define.h:3);
+ *     here it is the explicit first argument `n_s`;
+ *   - arrays that are assumed-shape in the reference carry their extent explicitly
+ *     (nv_aux);
+ *   - `stream` is a hipStream_t (NULL = default stream); work is enqueued, not waited
+ *     for, except where an output is returned to the host;
+ *   - the return value is 0 on success, a positive hipError_t on a HIP failure, or a
+ *     negative STREAMS_E_* code for an invalid argument.
+ * Indices of conserved / auxiliary variables follow index_define.h:
+ *   w:     1..N_S = rho*Y_s, N_S+1..N_S+4 = rho*u, rho*v, rho*w, rho*E
+ *   w_aux: 1..N_S = Y_s, N_S+1.. = u, v, w, h, T, p, c, mu, Ducros, div/3, rho, Z, k,
+ *          N_S+13+s = rho*D_s.
+ */
+#ifndef STREAMS_HIP_H
+#define STREAMS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STREAMS_E_NSPECIES   (-1)  /* N_S outside the compiled set            */
+#define STREAMS_E_DIMS       (-2)  /* inconsistent sizes or index ranges      */
+#define STREAMS_E_OPTION     (-3)  /* unsupported numerical option            */
+#define STREAMS_E_NOMEM      (-4)  /* device allocation for scratch failed    */
+
+/* Version / capability query: returns the number of compiled N_S values and writes up to
+ * `cap` of them into `list`. */
+int streams_species_supported(int *list, int cap);
+
+/* ---- auxiliary variables (a1, a2) ------------------------------------------------- */
+
+/* eval_aux_cuf, kernels_gpu.F90:6154-6308.  Clips rho*Y_s >= 0 in w (side effect), writes
+ * Y, u, v, w, h, T (Newton on e from the previous w_aux T), p, c, mu (Wilke), k (Mathur),
+ * rho*D_s (Bird), Z over the box [istart:iend]x[jstart:jend]x[kstart:kend]. */
+int streams_eval_aux(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux,
+                     int istart, int iend, int jstart, int jend, int kstart, int kend,
+                     double *w, double *w_aux, double p0, double t_min_tab, double dt_tab,
+                     double r_univ, const double *rgas, const double *cv_coeff,
+                     const double *cp_coeff, int nsetcv, const double *trange,
+                     int indx_cp_l, int indx_cp_r, double tol_iter_nr, void *stream,
+                     const double *mw, const double *mwinv, const double *visc_species,
+                     const double *lambda_species, const double *diffbin_species,
+                     int num_t_tab, int n_eoi, const double *aw_eoi, const int *nainsp,
+                     const double *beta0, const double *coeff_eoi);
+
+/* eval_aux2_cuf, kernels_gpu.F90:6310-6382: Ducros sensor and div/3 on the interior. */
+int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
+                      double *w_aux, const double *coeff_deriv1, const double *dcsidx,
+                      const double *detady, const double *dzitdz, double eps_sensor,
+                      int sensor_type, void *stream);
+
+/* ---- ghost cells and halo exchange (a3, a11) --------------------------------------- */
+
+/* bcextr_var_cuf, kernels_gpu.F90:5147-5207: zero-order extrapolation of one variable
+ * (w_var points at its first element) on all six faces. */
+int streams_bcextr_var(int nx, int ny, int nz, int ng, double *w_var, void *stream);
+
+/* base_gpu%bcswap / bcswap_var, base_gpu.F90:1132-1264, for one rank: a face whose
+ * neighbour is the rank itself (periodic axis) receives the loopback copy of the
+ * opposite ng-deep interior slab; a face with no neighbour is left alone.  All slabs are
+ * read before any ghost is written, as pack -> copy -> unpack does.  periodic[3] gives
+ * the x, y, z flags.  nv = number of consecutive variables starting at w. */
+int streams_bcswap(int nx, int ny, int nz, int ng, int nv, double *w,
+                   const int *periodic, void *stream);
+
+/* bcextr_cuf, kernels_gpu.F90:5209-5288 (face ilat = 1..6). */
+int streams_bcextr(int ilat, int nx, int ny, int nz, int ng, int nv, double *w, void *stream);
+
+/* bcwall_ad_cuf, kernels_gpu.F90:5749-5852: reflecting adiabatic wall on face ilat. */
+int streams_bcwall_ad(int n_s, int ilat, int nx, int ny, int nz, int ng, double *w,
+                      void *stream);
+
+/* ---- convective flux (a4, a5) ------------------------------------------------------ */
+
+/* euler_x (multideal_gpu.F90:948-1002): euler_x_fluxes_hybrid_kernel
+ * (kernels_gpu.F90:88-425; flux_splitting=1: the Rusanov variant :427-730) on faces
+ * istart_face..iend_face, then euler_x_update_cuf (:732-750) fl += (f_i - f_{i-1}) dcsidx
+ * on cells eul_imin..eul_imax.  fhat is scratch: its content afterwards is unspecified. */
+int streams_euler_x(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
+                    int istart_face, int iend_face, int lmax_base, int nkeep,
+                    const double *rgas, const double *w_aux, const double *coeff_deriv1,
+                    const double *dcsidx, double *fhat, int force_zero_flux_min,
+                    int force_zero_flux_max, int weno_scheme, int weno_version,
+                    double sensor_threshold, int weno_size, const double *cv_coeff,
+                    const double *cp_coeff, int nsetcv, const double *trange,
+                    int indx_cp_l, int indx_cp_r, const int *ep_ord_change,
+                    double tol_iter_nr, double rho0, double u0, int flux_splitting,
+                    int eul_imin, int eul_imax, double *fl, void *stream);
+
+/* euler_y (multideal_gpu.F90:1004-1045): euler_y_hybrid_kernel (kernels_gpu.F90:1427-1759)
+ * on faces eul_jmin-1..eul_jmax and euler_y_update_cuf (:752-771). */
+int streams_euler_y(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
+                    int eul_jmin, int eul_jmax, int lmax_base, int nkeep,
+                    const double *rgas, const double *w_aux, double *fl,
+                    const double *coeff_deriv1, const double *detady, double *fhat,
+                    int force_zero_flux_min, int force_zero_flux_max, int weno_scheme,
+                    int weno_version, double sensor_threshold, int weno_size,
+                    const double *cv_coeff, const double *cp_coeff, int nsetcv,
+                    const double *trange, int indx_cp_l, int indx_cp_r,
+                    const int *ep_ord_change, double tol_iter_nr, double rho0, double u0,
+                    int flux_splitting, void *stream);
+
+/* euler_z (multideal_gpu.F90:1047-1090): euler_z_hybrid_kernel (kernels_gpu.F90:795-1123)
+ * on faces eul_kmin-1..eul_kmax and euler_z_update_cuf (:773-793). */
+int streams_euler_z(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
+                    int eul_kmin, int eul_kmax, int lmax_base, int nkeep,
+                    const double *rgas, const double *w_aux, double *fl,
+                    const double *coeff_deriv1, const double *dzitdz, double *fhat,
+                    int force_zero_flux_min, int force_zero_flux_max, int weno_scheme,
+                    int weno_version, double sensor_threshold, int weno_size,
+                    const double *cv_coeff, const double *cp_coeff, int nsetcv,
+                    const double *trange, int indx_cp_l, int indx_cp_r,
+                    const int *ep_ord_change, double tol_iter_nr, double rho0, double u0,
+                    int flux_splitting, void *stream);
+
+/* ---- viscous and diffusive fluxes (a6) --------------------------------------------- */
+
+/* visflx_x_cuf / _y / _z, kernels_gpu.F90:3658-4031 (conservative_viscous=1, mode 1):
+ * staggered-face stress, conduction, Hirschfelder-Curtiss diffusion with correction
+ * velocity and enthalpy diffusion, then the face-difference update of fl.  `coord` is
+ * x (1-ng:nx+ng+1), y or z (1-ng:n+ng).  fhat is scratch. */
+int streams_visflx_x(int n_s, int nx, int ny, int nz, int nv, int nv_aux, int ng,
+                     const double *x, const double *w_aux, double *fl, double *fhat,
+                     int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
+                     const double *trange, const double *rgas, double r_univ, void *stream);
+int streams_visflx_y(int n_s, int nx, int ny, int nz, int nv, int nv_aux, int ng,
+                     const double *y, const double *w_aux, double *fl, double *fhat,
+                     int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
+                     const double *trange, const double *rgas, double r_univ, void *stream);
+int streams_visflx_z(int n_s, int nx, int ny, int nz, int nv, int nv_aux, int ng,
+                     const double *z, const double *w_aux, double *fl, double *fhat,
+                     int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
+                     const double *trange, const double *rgas, double r_univ, void *stream);
+
+/* visflx_reduced_cuf, kernels_gpu.F90:3539-3656 (mode 2): non-Laplacian stress terms and
+ * viscous work. */
+int streams_visflx_reduced(int n_s, int nx, int ny, int nz, int ng, int nv_aux,
+                           int visc_order, const double *w_aux, double *fl,
+                           const double *coeff_deriv1, const double *dcsidx,
+                           const double *detady, const double *dzitdz, void *stream);
+
+/* ---- chemistry (a7, a8) ------------------------------------------------------------ */
+
+/* compute_chemistry_cuf, kernels_gpu.F90:9503-9674: explicit finite-rate source,
+ * fl(1:N_S) -= omega_s W_s; clips w, rewrites w_aux Y and T. */
+int streams_compute_chemistry(int n_s, int nx, int ny, int nz, int nv, int nv_aux, int ng,
+                              int nreactions, double *w, double *w_aux, double *fl,
+                              const double *mw, const double *arr_a, const double *arr_b,
+                              const double *arr_ea, const double *tb_eff,
+                              const double *falloff_coeffs, const int *reac_ty,
+                              const int *isrev, const double *r_coeffs,
+                              const double *p_coeffs, const double *kc_tab, int num_t_tab,
+                              double t_min_tab, double dt_tab, double r_univ, int indx_cp_l,
+                              int indx_cp_r, const double *cv_coeff, int nsetcv,
+                              const double *trange, double tol_iter_nr, void *stream);
+
+/* rosenbrock_kernel, kernels_gpu.F90:8928-9501 (launched by rosenbrock,
+ * multideal_gpu.F90:694-767): per cell adaptive 4-stage Rosenbrock from time_start to
+ * time_end with a central-difference Jacobian and partially pivoted LU, all FP64.
+ * enable_pasr must be 0 (the PaSR closure belongs to the LES path). */
+int streams_rosenbrock(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
+                       double *w, double *w_aux, const double *fl, const double *mw,
+                       int nreactions, const double *arr_a, const double *arr_b,
+                       const double *arr_ea, const double *tb_eff,
+                       const double *falloff_coeffs, const int *reac_ty, const int *isrev,
+                       const double *r_coeffs, const double *p_coeffs,
+                       const double *kc_tab, int num_t_tab, double t_min_tab,
+                       double dt_tab, double r_univ, double tol_iter_nr,
+                       const double *cv_coeff, int nsetcv, const double *trange,
+                       int indx_cp_l, int indx_cp_r, double dttry, double time_start,
+                       double time_end, int maxsteps, int maxtry, double tol,
+                       int simpler_splitting, int enable_pasr, double les_c_yoshi,
+                       double les_c_mix, double les_c_eps, const double *dcsidx,
+                       const double *detady, const double *dzitdz, void *stream);
+
+/* ---- Runge-Kutta bookkeeping and time step (a10, a12) ------------------------------ */
+
+/* init_flux_cuf (kernels_gpu.F90:28-45): fln = -rhodt*fl; fl = 0. */
+int streams_init_flux(int nx, int ny, int nz, int nv, double *fl, double *fln,
+                      double rhodt, void *stream);
+/* update_flux_cuf (:3297-3314): fln -= gamdt*fl. */
+int streams_update_flux(int nx, int ny, int nz, int nv, const double *fl, double *fln,
+                        double gamdt, void *stream);
+/* update_field_cuf (:3336-3355): w += fln where fluid_mask == 0 (fluid_mask may be NULL
+ * for an all-fluid domain). */
+int streams_update_field(int nx, int ny, int nz, int ng, int nv, double *w,
+                         const double *fln, const int *fluid_mask, void *stream);
+
+/* compute_dt_cuf (kernels_gpu.F90:6075-6152): the nine inverse-time-step maxima
+ * (convective x,y,z; viscous x,y,z; conductive x,y,z) written to host `dtmax9`; the call
+ * waits for the stream.  fluid_mask may be NULL. */
+int streams_compute_dt(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux,
+                       const double *dcsidx, const double *detady, const double *dzitdz,
+                       const double *dcsidxs, const double *detadys, const double *dzitdzs,
+                       const double *w, const double *w_aux, double *dtmax9,
+                       int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
+                       const double *trange, const int *fluid_mask, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STREAMS_HIP_H */

@@ -1,0 +1,135 @@
+"""ctypes binding of ``libstreams_hip.so`` (the C-ABI declared in ``include/streams_hip.h``).
+
+``SIGNATURES`` lists every entry point with its argument kinds in header order; the same
+table drives the binding here and any other library that exports the same entry points
+under another prefix.  Argument kinds: ``i`` int, ``d`` double, ``p`` pointer (torch
+tensor, numpy array, integer address or None), ``s`` stream, ``P`` host pointer to an
+output array.
+
+Loading fails loudly: there is no fallback path when the library is missing or a GPU is
+absent.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+from typing import Dict, List, Tuple
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIBPATH = PKG / "lib" / "libstreams_hip.so"
+
+STATUS = {-1: "N_S not compiled", -2: "inconsistent sizes", -3: "unsupported option", -4: "allocation failed"}
+
+_ARGS = {
+    "eval_aux": "iiiiiiiiiiiii" "pp" "dddd" "ppp" "ip" "iid" "s" "ppppp" "i" "i" "pppp",
+    "eval_aux2": "iiiiiii" "p" "pppp" "d" "i" "s",
+    "bcextr_var": "iiii" "p" "s",
+    "bcswap": "iiiii" "p" "p" "s",
+    "bcextr": "iiiiii" "p" "s",
+    "bcwall_ad": "iiiiii" "p" "s",
+    "euler_x": "iiiiiii" "iiii" "pppp" "p" "ii" "ii" "d" "i" "pp" "i" "p" "ii" "p" "ddd" "i" "ii" "p" "s",
+    "euler_y": "iiiiiii" "iiii" "pp" "p" "pp" "p" "ii" "ii" "d" "i" "pp" "i" "p" "ii" "p" "ddd" "i" "s",
+    "euler_z": "iiiiiii" "iiii" "pp" "p" "pp" "p" "ii" "ii" "d" "i" "pp" "i" "p" "ii" "p" "ddd" "i" "s",
+    "visflx_x": "iiiiiii" "p" "p" "pp" "ii" "p" "i" "pp" "d" "s",
+    "visflx_y": "iiiiiii" "p" "p" "pp" "ii" "p" "i" "pp" "d" "s",
+    "visflx_z": "iiiiiii" "p" "p" "pp" "ii" "p" "i" "pp" "d" "s",
+    "visflx_reduced": "iiiiiii" "p" "p" "pppp" "s",
+    "compute_chemistry": "iiiiiiii" "ppp" "p" "pppp" "p" "pp" "ppp" "i" "ddd" "ii" "p" "i" "p" "d" "s",
+    "rosenbrock": "iiiiiii" "ppp" "p" "i" "pppp" "p" "pp" "ppp" "i" "dddd" "p" "i" "p" "ii" "ddd" "iid" "ii"
+                  "ddd" "ppp" "s",
+    "init_flux": "iiii" "pp" "d" "s",
+    "update_flux": "iiii" "pp" "d" "s",
+    "update_field": "iiiii" "ppp" "s",
+    "compute_dt": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
+}
+
+SIGNATURES: Dict[str, str] = dict(_ARGS)
+
+_CT = {"i": ctypes.c_int, "d": ctypes.c_double, "p": ctypes.c_void_p, "s": ctypes.c_void_p,
+       "P": ctypes.c_void_p}
+
+
+class StreamsError(RuntimeError):
+    pass
+
+
+def as_pointer(x):
+    """Address of a torch tensor / numpy array / integer (None -> NULL)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        return x.ctypes.data
+    raise TypeError(f"cannot pass {type(x)} as a pointer")
+
+
+class Library:
+    """A shared library exporting the C-ABI under ``prefix`` (``streams_`` for the product)."""
+
+    def __init__(self, path: os.PathLike, prefix: str, mode: int = ctypes.RTLD_GLOBAL):
+        self.path = str(path)
+        self.prefix = prefix
+        self.lib = ctypes.CDLL(self.path, mode=mode)
+        self._fn = {}
+        for name, kinds in SIGNATURES.items():
+            sym = getattr(self.lib, prefix + name, None)
+            if sym is None:
+                continue
+            sym.restype = ctypes.c_int
+            sym.argtypes = [_CT[k] for k in kinds]
+            self._fn[name] = (sym, kinds)
+
+    def has(self, name: str) -> bool:
+        return name in self._fn
+
+    def exported(self) -> List[str]:
+        return sorted(self._fn)
+
+    def call(self, name: str, *args):
+        sym, kinds = self._fn[name]
+        if len(args) != len(kinds):
+            raise TypeError(f"{name}: expected {len(kinds)} arguments, got {len(args)}")
+        conv = []
+        for a, k in zip(args, kinds):
+            if k in "psP":
+                conv.append(as_pointer(a))
+            elif k == "i":
+                conv.append(int(a))
+            else:
+                conv.append(float(a))
+        rc = sym(*conv)
+        if rc != 0:
+            msg = STATUS.get(rc, f"HIP error {rc}")
+            raise StreamsError(f"{self.prefix}{name} returned {rc}: {msg}")
+        return rc
+
+
+_LIB = None
+
+
+def load(path: os.PathLike = LIBPATH) -> Library:
+    """Load libstreams_hip.so; raises if the library has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not pathlib.Path(path).exists():
+            raise StreamsError(f"{path} is missing: build it with prop-streams2_amd/build.py "
+                               "(there is no CPU fallback for the hot path)")
+        _LIB = Library(path, "streams_")
+        missing = sorted(set(SIGNATURES) - set(_LIB.exported()))
+        if missing:
+            raise StreamsError(f"libstreams_hip.so lacks entry points: {missing}")
+    return _LIB
+
+
+def species_supported() -> Tuple[int, ...]:
+    lib = load()
+    sym = lib.lib.streams_species_supported
+    sym.restype = ctypes.c_int
+    sym.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    buf = (ctypes.c_int * 64)()
+    n = sym(buf, 64)
+    return tuple(buf[i] for i in range(min(n, 64)))

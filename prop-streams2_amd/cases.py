@@ -1,0 +1,142 @@
+"""Seeded synthetic inputs for the measured configurations.
+
+* ``h2_air_mechanism`` — the 9-species H2/O2/N2 set: the laminar-premixed-flame
+  mechanism of the reference's examples (11 species, 21 reactions incl. three-body and
+  Troe falloff) with Ar and He removed (they appear there only as collision partners).
+* ``turbulence_box`` — periodic reacting decaying-turbulence cube: lean H2-air (phi=0.5,
+  X_H2:X_O2:X_N2 = 1:1:3.76), uniform T and p, solenoidal velocity from seeded random
+  phases with a Passot-Pouquet spectrum E(k) ~ (k/k0)^4 exp(-2 (k/k0)^2), scaled to a
+  turbulent Mach number M_t = sqrt(<u.u>)/c.  Domain side 2*pi mm.
+* ``reactor_batch`` — independent 0-D reactors: T log-uniform, p log-uniform, Y from a
+  seeded Dirichlet(1) over the species, rho from the equation of state, at rest.
+* ``shock_tube`` — the reactive shock-tube initial state of init_reactive_tube
+  (multideal.F90:5305-5381) extruded along x.
+
+Arrays are returned as ``w[m, k, j, i]`` (C order, i.e. the reference's Fortran layout
+``w(i, j, k, m)``) including the ng ghost layers, which are filled with the periodic
+extension for periodic axes.
+"""
+from __future__ import annotations
+
+import math
+import pathlib
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .mechanism import Mechanism, R_UNIV, Tables
+
+MECH_DIR = pathlib.Path(__file__).resolve().parents[1] / "tests" / "golden" / "mechanisms"
+H2_AIR_9 = ["H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"]
+
+
+def mechanism_path(name: str) -> pathlib.Path:
+    return MECH_DIR / name
+
+
+def h2_air_mechanism() -> Mechanism:
+    return Mechanism(str(mechanism_path("h2_premix_11sp.yaml")), species=H2_AIR_9)
+
+
+def mole_to_mass(tables: Tables, x: dict) -> np.ndarray:
+    xs = np.array([x.get(n, 0.0) for n in tables.names])
+    y = xs * tables.mw
+    return y / y.sum()
+
+
+def _pad_periodic(a: np.ndarray, ng: int) -> np.ndarray:
+    return np.pad(a, [(0, 0)] + [(ng, ng)] * (a.ndim - 1), mode="wrap")
+
+
+def solenoidal_field(n: int, seed: int, k0: float = 4.0) -> np.ndarray:
+    """Random-phase solenoidal velocity (3, n, n, n) with unit rms speed sqrt(<u.u>)=1."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    kz, ky, kx = np.meshgrid(k1, k1, kr, indexing="ij")
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    kk[0, 0, 0] = 1.0
+    spec = (kk / k0) ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
+    amp = np.sqrt(spec / (4.0 * math.pi * kk * kk))
+    amp[0, 0, 0] = 0.0
+    shape = kk.shape
+    xi = [(rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * amp for _ in range(3)]
+    # project onto the plane normal to k
+    kdot = (kx * xi[0] + ky * xi[1] + kz * xi[2]) / (kk * kk)
+    uh = [xi[0] - kx * kdot, xi[1] - ky * kdot, xi[2] - kz * kdot]
+    u = np.stack([np.fft.irfftn(c, s=(n, n, n), axes=(0, 1, 2)) for c in uh])    # axes (k, j, i)
+    u -= u.mean(axis=(1, 2, 3), keepdims=True)
+    urms = math.sqrt(float((u * u).sum(axis=0).mean()))
+    return u / urms
+
+
+def turbulence_box(tables: Tables, n: int, seed: int = 2024, ng: int = 3, temperature: float = 1200.0,
+                   pressure: float = 101325.0, mach_t: float = 0.3, k0: float = 4.0,
+                   x: Optional[dict] = None):
+    """Conserved variables (nv, n+2ng, n+2ng, n+2ng) and the temperature guess."""
+    x = x or {"H2": 1.0, "O2": 1.0, "N2": 3.76}
+    y = mole_to_mass(tables, x)
+    rmix = tables.rmix(y)
+    rho = pressure / (rmix * temperature)
+    cp = tables.cp_from_T(temperature, y)
+    c = math.sqrt(cp / (cp - rmix) * rmix * temperature)
+    vel = solenoidal_field(n, seed, k0) * (mach_t * c)
+    e = tables.e_from_T(temperature, y)
+    ns = tables.n_species
+    w = np.empty((ns + 4, n, n, n))
+    for s in range(ns):
+        w[s] = rho * y[s]
+    w[ns + 0] = rho * vel[0]      # vel axes are (k, j, i): component order x, y, z
+    w[ns + 1] = rho * vel[1]
+    w[ns + 2] = rho * vel[2]
+    w[ns + 3] = rho * (e + 0.5 * (vel * vel).sum(axis=0))
+    wp = _pad_periodic(w, ng)
+    tg = np.full(wp.shape[1:], temperature)
+    info = dict(rho0=rho, t0=temperature, p0=pressure, c0=c, u_rms=mach_t * c, length=2 * math.pi * 1e-3, y=y)
+    return wp, tg, info
+
+
+def reactor_batch(tables: Tables, n: int, seed: int = 7, t_range=(800.0, 2500.0),
+                  p_range=(0.5 * 101325.0, 20 * 101325.0), alpha: float = 1.0):
+    """n independent reactors: SoA conserved state w[nv, n] and the T guess."""
+    rng = np.random.default_rng(seed)
+    ns = tables.n_species
+    T = np.exp(rng.uniform(math.log(t_range[0]), math.log(t_range[1]), n))
+    p = np.exp(rng.uniform(math.log(p_range[0]), math.log(p_range[1]), n))
+    Y = rng.dirichlet(np.full(ns, alpha), size=n)
+    w = np.zeros((ns + 4, n))
+    for b in range(n):
+        y = Y[b]
+        rho = p[b] / (tables.rmix(y) * T[b])
+        w[:ns, b] = rho * y
+        w[ns + 3, b] = rho * tables.e_from_T(T[b], y)
+    return w, T.copy(), dict(T=T, p=p, Y=Y)
+
+
+def shock_tube(tables: Tables, nx: int, init_y: Sequence[float], rho0: float, p0: float,
+               ng: int = 3, length: float = 0.12, ny: int = 1, nz: int = 1):
+    """init_reactive_tube: quiescent (rho0, p0) for x < x(nx)/2, driven state beyond."""
+    ns = tables.n_species
+    y = np.asarray(init_y, dtype=np.float64)
+    rmix = tables.rmix(y)
+    t0 = p0 / rho0 / rmix
+    e0 = tables.e_from_T(t0, y)
+    win = np.zeros(ns + 4)
+    win[:ns] = rho0 * y
+    win[ns + 3] = rho0 * e0
+    rho_t, u_t, p_t = 0.18075, -487.34, 35594.0
+    t_t = p_t / rmix / rho_t
+    e_t = tables.e_from_T(t_t, y)
+    wt = np.zeros(ns + 4)
+    wt[:ns] = rho_t * y
+    wt[ns] = rho_t * u_t
+    wt[ns + 3] = rho_t * (e_t + 0.5 * u_t * u_t)
+    dx = length / (nx - 1)
+    xs = (np.arange(1, nx + 1) - 1) * dx
+    w = np.zeros((ns + 4, nz, ny, nx))
+    left = xs < 0.5 * xs[-1]
+    w[:, :, :, left] = win[:, None, None, None]
+    w[:, :, :, ~left] = wt[:, None, None, None]
+    wp = np.pad(w, [(0, 0), (ng, ng), (ng, ng), (ng, ng)], mode="edge")
+    tg = np.where(np.pad(np.broadcast_to(left, (nz, ny, nx)), [(ng, ng)] * 3, mode="edge"), t0, t_t)
+    return wp, tg, dict(t0=t0, t_tube=t_t, rho0=rho0, p0=p0)

@@ -1,0 +1,248 @@
+// aux.hip — auxiliary variables: thermo state, mixture transport, shock sensor.
+//
+// eval_aux: one thread per point of the requested box; the box is flattened with i
+// fastest so that every load and store of a wave touches 64 consecutive doubles of one
+// variable plane (coalesced 512-byte segments).  Pairwise Wilke constants that depend on
+// the molar masses only are built once per workgroup in LDS.
+#include "common.h"
+
+namespace streams {
+
+struct AuxTables {
+  const double *rgas, *mw, *mwinv, *visc, *lambda, *diffbin;
+  int num_t_tab;
+  double t_min_tab, dt_tab, p0, r_univ;
+  int n_eoi;
+  const double *aw_eoi, *beta0, *coeff_eoi;
+  const int *nainsp;
+};
+
+template <int NS>
+__global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int jstart, int kstart,
+                                                       int ni, int nj, long npts, double *__restrict__ w,
+                                                       double *__restrict__ w_aux, Thermo th,
+                                                       AuxTables tb, double tol_iter_nr) {
+  using X = Idx<NS>;
+  __shared__ double c_inv[NS * NS];  // 1/sqrt(1 + W_l/W_m)
+  __shared__ double c_pow[NS * NS];  // (W_m/W_l)^(1/4)
+  for (int q = threadIdx.x; q < NS * NS; q += blockDim.x) {
+    int l = q % NS, m = q / NS;
+    c_inv[q] = 1.0 / sqrt(1.0 + tb.mw[l] * tb.mwinv[m]);
+    c_pow[q] = pow(tb.mw[m] * tb.mwinv[l], 0.25);
+  }
+  __syncthreads();
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  int i = istart + (int)(t % ni);
+  long r = t / ni;
+  int j = jstart + (int)(r % nj);
+  int k = kstart + (int)(r / nj);
+  const long pt = b.p(i, j, k), ps = b.pstride();
+
+  double yy[NS];
+  double rho = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double v = w[pt + ps * s];
+    if (!(v > 0.0)) {               // max(rho*Y, 0): written back only when clipped
+      v = 0.0;
+      w[pt + ps * s] = 0.0;
+    }
+    yy[s] = v;
+    rho = (s == 0) ? v : rho + v;
+  }
+  const double ri = 1.0 / rho;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    yy[s] = yy[s] * ri;
+    w_aux[pt + ps * s] = yy[s];
+  }
+  const double rhou = w[pt + ps * (X::I_U - 1)], rhov = w[pt + ps * (X::I_V - 1)];
+  const double rhow = w[pt + ps * (X::I_W - 1)], rhoe = w[pt + ps * (X::I_E - 1)];
+  const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+  const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+  const double ee = rhoe * ri - qq;
+  const double tt = temperature_from_e<NS>(ee, w_aux[pt + ps * (X::J_T - 1)], th, tol_iter_nr, yy);
+  double rmix = tb.rgas[0] * yy[0];
+#pragma unroll
+  for (int s = 1; s < NS; ++s) rmix = rmix + tb.rgas[s] * yy[s];
+  const double cploc = mixture_cp<NS>(tt, th, yy);
+  const double gamloc = cploc / (cploc - rmix);
+  const double pp = rho * tt * rmix;
+  const double c = sqrt(gamloc * pp * ri);
+  w_aux[pt + ps * (X::J_R - 1)] = rho;
+  w_aux[pt + ps * (X::J_U - 1)] = uu;
+  w_aux[pt + ps * (X::J_V - 1)] = vv;
+  w_aux[pt + ps * (X::J_W - 1)] = ww;
+  w_aux[pt + ps * (X::J_H - 1)] = (rhoe + pp) * ri;
+  w_aux[pt + ps * (X::J_T - 1)] = tt;
+  w_aux[pt + ps * (X::J_P - 1)] = pp;
+  w_aux[pt + ps * (X::J_C - 1)] = c;
+  // Bilger mixture fraction (kernels_gpu.F90:6228-6243)
+  double zz = 0.5;
+  if (tb.n_eoi != 1) {
+    double beta = 0.0;
+    for (int le = 0; le < tb.n_eoi; ++le) {
+      double yatm = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (yy[s] > 1e-14) yatm = yatm + tb.nainsp[s + NS * le] * tb.aw_eoi[le] * yy[s] * tb.mwinv[s];
+      beta = beta + tb.coeff_eoi[le] * yatm / tb.aw_eoi[le];
+    }
+    zz = (beta - tb.beta0[1]) / (tb.beta0[0] - tb.beta0[1]);
+  }
+  w_aux[pt + ps * (X::J_Z - 1)] = zz;
+  // table row
+  int itt = (int)((tt - tb.t_min_tab) / tb.dt_tab) + 1;
+  itt = itt < 1 ? 1 : itt;
+  itt = itt > tb.num_t_tab ? tb.num_t_tab : itt;
+  const double tloc = tb.t_min_tab + (itt - 1) * tb.dt_tab;
+  const double dtt = (tt - tloc) / tb.dt_tab;
+  const long nt = tb.num_t_tab + 1;
+  // Mathur conductivity
+  const double mwmixt = tb.r_univ / rmix;
+  double xs[NS];
+  double k1 = 0.0, k2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double *lp = tb.lambda + (itt - 1) + nt * s;
+    double l0 = lp[0], l1 = lp[1];
+    double klsp = l0 + (l1 - l0) * dtt;
+    xs[s] = yy[s] * mwmixt * tb.mwinv[s];
+    k1 = k1 + xs[s] * klsp;
+    k2 = k2 + xs[s] / klsp;
+  }
+  const double k_cond = 0.5 * (k1 + 1.0 / k2);
+  // Wilke viscosity: sqrt(mu_m/mu_l) evaluated as sqrt(mu_m) * (1/sqrt(mu_l))
+  double mus[NS], smu[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double *vp = tb.visc + (itt - 1) + nt * s;
+    double v0 = vp[0], v1 = vp[1];
+    mus[s] = v0 + (v1 - v0) * dtt;
+    smu[s] = sqrt(mus[s]);
+  }
+  double mu = 0.0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double isl = 1.0 / smu[l];
+    double den = 0.0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      double t1 = 1.0 + (smu[m] * isl) * c_pow[l + NS * m];
+      double phi = c_inv[l + NS * m] * (t1 * t1);
+      den = den + yy[m] * phi * tb.mwinv[m];
+    }
+    mu = mu + mus[l] * yy[l] * tb.mwinv[l] / den;
+  }
+  mu = sqrt(8.0) * mu;
+  w_aux[pt + ps * (X::J_MU - 1)] = mu;
+  w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond;
+  // Bird species diffusion into the mixture
+  const double fac = tb.p0 / pp * rho;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    double dden = 0.0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      if (m != l) {
+        const double *dp = tb.diffbin + (itt - 1) + nt * (l + NS * m);
+        double d0 = dp[0], d1 = dp[1];
+        double dij = d0 + (d1 - d0) * dtt;
+        dden = dden + xs[m] / dij;
+      }
+    }
+    w_aux[pt + ps * (X::J_D_START + l)] = (dden > 1.0e-15) ? (1.0 - yy[l]) / dden * fac : 0.0;
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(256) eval_aux2_kernel(Box b, int lmax, double *__restrict__ w_aux,
+                                                        const double *__restrict__ cd1,
+                                                        const double *__restrict__ dcsidx,
+                                                        const double *__restrict__ detady,
+                                                        const double *__restrict__ dzitdz,
+                                                        double eps_sensor, int sensor_type) {
+  using X = Idx<NS>;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long n = (long)b.nx * b.ny * b.nz;
+  if (t >= n) return;
+  int i = 1 + (int)(t % b.nx);
+  long r = t / b.nx;
+  int j = 1 + (int)(r % b.ny);
+  int k = 1 + (int)(r / b.ny);
+  const long pt = b.p(i, j, k), ps = b.pstride();
+  const long sx = 1, sy = b.px(), sz = b.px() * b.py();
+  const double *U = w_aux + ps * (X::J_U - 1), *V = w_aux + ps * (X::J_V - 1), *Wv = w_aux + ps * (X::J_W - 1);
+  double ux = 0, vx = 0, wx = 0, uy = 0, vy = 0, wy = 0, uz = 0, vz = 0, wz = 0;
+  for (int l = 1; l <= lmax; ++l) {
+    double ccl = cd1[(l - 1) + 4 * (lmax - 1)];
+    ux = ux + ccl * (U[pt + l * sx] - U[pt - l * sx]);
+    vx = vx + ccl * (V[pt + l * sx] - V[pt - l * sx]);
+    wx = wx + ccl * (Wv[pt + l * sx] - Wv[pt - l * sx]);
+    uy = uy + ccl * (U[pt + l * sy] - U[pt - l * sy]);
+    vy = vy + ccl * (V[pt + l * sy] - V[pt - l * sy]);
+    wy = wy + ccl * (Wv[pt + l * sy] - Wv[pt - l * sy]);
+    uz = uz + ccl * (U[pt + l * sz] - U[pt - l * sz]);
+    vz = vz + ccl * (V[pt + l * sz] - V[pt - l * sz]);
+    wz = wz + ccl * (Wv[pt + l * sz] - Wv[pt - l * sz]);
+  }
+  const double dx = dcsidx[i - 1], dy = detady[j - 1], dz = dzitdz[k - 1];
+  ux *= dx; vx *= dx; wx *= dx;
+  uy *= dy; vy *= dy; wy *= dy;
+  uz *= dz; vz *= dz; wz *= dz;
+  const double div = ux + vy + wz;
+  const double ox = wy - vz, oy = uz - wx, oz = vx - uy;
+  const double omod2 = ox * ox + oy * oy + oz * oz;
+  double s = -div / sqrt(omod2 + div * div + eps_sensor);
+  if (sensor_type != 0 && s < 0.0) s = 0.0;
+  w_aux[pt + ps * (X::J_DUC - 1)] = s * s;
+  w_aux[pt + ps * (X::J_DIV - 1)] = div / 3.0;
+}
+
+}  // namespace streams
+
+using namespace streams;
+
+extern "C" int streams_eval_aux(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux,
+                                int istart, int iend, int jstart, int jend, int kstart, int kend,
+                                double *w, double *w_aux, double p0, double t_min_tab, double dt_tab,
+                                double r_univ, const double *rgas, const double *cv_coeff,
+                                const double *cp_coeff, int nsetcv, const double *trange,
+                                int indx_cp_l, int indx_cp_r, double tol_iter_nr, void *stream,
+                                const double *mw, const double *mwinv, const double *visc_species,
+                                const double *lambda_species, const double *diffbin_species,
+                                int num_t_tab, int n_eoi, const double *aw_eoi, const int *nainsp,
+                                const double *beta0, const double *coeff_eoi) {
+  if (nv != n_s + 4 || nv_aux < 2 * n_s + 13) return STREAMS_E_DIMS;
+  if (iend < istart || jend < jstart || kend < kstart) return 0;
+  if (istart < 1 - ng || iend > nx + ng || jstart < 1 - ng || jend > ny + ng || kstart < 1 - ng ||
+      kend > nz + ng)
+    return STREAMS_E_DIMS;
+  if (num_t_tab < 1) return STREAMS_E_DIMS;
+  Box b{nx, ny, nz, ng};
+  Thermo th{cv_coeff, cp_coeff, trange, nsetcv, indx_cp_l, indx_cp_r};
+  AuxTables tb{rgas, mw, mwinv, visc_species, lambda_species, diffbin_species, num_t_tab, t_min_tab,
+               dt_tab, p0, r_univ, n_eoi, aw_eoi, beta0, coeff_eoi, nainsp};
+  int ni = iend - istart + 1, nj = jend - jstart + 1, nk = kend - kstart + 1;
+  long npts = (long)ni * nj * nk;
+  STREAMS_DISPATCH_NS(n_s, (eval_aux_kernel<NS><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
+                                b, istart, jstart, kstart, ni, nj, npts, w, w_aux, th, tb, tol_iter_nr)));
+  return hip_status();
+}
+
+extern "C" int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
+                                 double *w_aux, const double *coeff_deriv1, const double *dcsidx,
+                                 const double *detady, const double *dzitdz, double eps_sensor,
+                                 int sensor_type, void *stream) {
+  if (nv_aux < 2 * n_s + 13) return STREAMS_E_DIMS;
+  int lmax = visc_order / 2;
+  if (lmax < 1 || lmax > 4 || lmax > ng) return STREAMS_E_OPTION;
+  Box b{nx, ny, nz, ng};
+  long n = (long)nx * ny * nz;
+  if (n == 0) return 0;
+  STREAMS_DISPATCH_NS(n_s, (eval_aux2_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+                                b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                sensor_type)));
+  return hip_status();
+}

@@ -1,0 +1,180 @@
+// common.h — shared device-side definitions for libstreams_hip (gfx950).
+//
+// Layouts follow the reference's Fortran arrays (column-major, first index fastest):
+//   padded field  a(1-ng:nx+ng, 1-ng:ny+ng, 1-ng:nz+ng, 1:nvar)
+//   interior field f(1:nx, 1:ny, 1:nz, 1:nv)
+//   cp/cv tables   c(indx_cp_l:indx_cp_r+2, N_S, nsetcv), trange(N_S, nsetcv+1)
+// N_S is a template parameter; the C-ABI dispatches on the runtime value.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/streams_hip.h"
+
+namespace streams {
+
+// --------------------------------------------------------------------- geometry
+struct Box {
+  int nx, ny, nz, ng;
+  __host__ __device__ inline long px() const { return nx + 2 * ng; }
+  __host__ __device__ inline long py() const { return ny + 2 * ng; }
+  __host__ __device__ inline long pz() const { return nz + 2 * ng; }
+  // padded point (i,j,k) with Fortran indices
+  __host__ __device__ inline long p(int i, int j, int k) const {
+    return (long)(i + ng - 1) + px() * ((long)(j + ng - 1) + py() * (long)(k + ng - 1));
+  }
+  __host__ __device__ inline long pstride() const { return px() * py() * pz(); }
+  // interior point
+  __host__ __device__ inline long f(int i, int j, int k) const {
+    return (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (long)(k - 1));
+  }
+  __host__ __device__ inline long fstride() const { return (long)nx * ny * nz; }
+};
+
+// index of variables (index_define.h), 1-based as in the reference
+template <int NS> struct Idx {
+  static constexpr int I_U = NS + 1, I_V = NS + 2, I_W = NS + 3, I_E = NS + 4;
+  static constexpr int J_U = NS + 1, J_V = NS + 2, J_W = NS + 3, J_H = NS + 4, J_T = NS + 5;
+  static constexpr int J_P = NS + 6, J_C = NS + 7, J_MU = NS + 8, J_DUC = NS + 9, J_DIV = NS + 10;
+  static constexpr int J_R = NS + 11, J_Z = NS + 12, J_K_COND = NS + 13, J_D_START = NS + 13;
+  static constexpr int NV = NS + 4;
+};
+
+// ------------------------------------------------------------------- thermo tables
+struct Thermo {
+  const double *cv_coeff, *cp_coeff, *trange;
+  int nsetcv, lo, hi;  // indx_cp_l, indx_cp_r
+  __device__ inline int ncol() const { return hi + 2 - lo + 1; }
+};
+
+template <int NS>
+__device__ inline double cpc(const double *tab, const Thermo &th, int l, int s, int m) {
+  // tab(l, s, m) with l in [lo, hi+2], s,m 1-based
+  return tab[(l - th.lo) + th.ncol() * ((s - 1) + NS * (m - 1))];
+}
+
+// Numerical-Recipes locate on trange(s,:), clamped to [1, nsetcv]
+// (kernels_gpu.F90:6400-6418)
+template <int NS>
+__device__ inline int locate_range(double tt, const Thermo &th, int s) {
+  if (th.nsetcv <= 1) return 1;
+  int jl = 0, ju = th.nsetcv + 2;
+  while (ju - jl > 1) {
+    int jm = (ju + jl) >> 1;
+    if (tt >= th.trange[(s - 1) + NS * (jm - 1)]) jl = jm; else ju = jm;
+  }
+  jl = jl < 1 ? 1 : jl;
+  return jl > th.nsetcv ? th.nsetcv : jl;
+}
+
+__device__ inline double ipow(double x, int n) {
+  // Fortran x**n with an integer exponent
+  if (n == 0) return 1.0;
+  bool neg = n < 0;
+  if (neg) n = -n;
+  double r = 1.0;
+  for (int q = 0; q < n; ++q) r *= x;
+  return neg ? 1.0 / r : r;
+}
+
+// Newton inversion e -> T (get_mixture_temperature_from_e_dev, kernels_gpu.F90:6532-6622)
+template <int NS>
+__device__ double temperature_from_e(double ee, double t_start, const Thermo &th,
+                                     double tol_iter_nr, const double *yy) {
+  int nrange[NS];
+#pragma unroll
+  for (int s = 1; s <= NS; ++s) nrange[s - 1] = locate_range<NS>(t_start, th, s);
+  double t_old = t_start, tt = t_start;
+  for (int iter = 1; iter <= 10; ++iter) {
+    if (th.nsetcv > 1) {
+#pragma unroll
+      for (int s = 1; s <= NS; ++s) {
+        int nr = nrange[s - 1];
+        if (t_old < th.trange[(s - 1) + NS * (nr - 1)]) nr = nr - 1;
+        else if (t_old > th.trange[(s - 1) + NS * nr]) nr = nr + 1;
+        nr = nr < 1 ? 1 : nr;
+        nrange[s - 1] = nr > th.nsetcv ? th.nsetcv : nr;
+      }
+    }
+    double sumb = 0.0;
+#pragma unroll
+    for (int s = 1; s <= NS; ++s) sumb = sumb + cpc<NS>(th.cv_coeff, th, th.hi + 1, s, nrange[s - 1]) * yy[s - 1];
+    double ebar = ee - sumb;
+    double den = 0.0, num = 0.0;
+    for (int l = th.lo; l <= th.hi; ++l) {
+      double t_pow = ipow(t_old, l);
+      double tden = t_pow, tnum;
+      if (l == -1) tnum = log(t_old);
+      else tnum = (t_old * t_pow) / (l + 1.0);
+      double cv_l = 0.0;
+#pragma unroll
+      for (int s = 1; s <= NS; ++s) cv_l = cv_l + cpc<NS>(th.cv_coeff, th, l, s, nrange[s - 1]) * yy[s - 1];
+      num = num + cv_l * tnum;
+      den = den + cv_l * tden;
+    }
+    tt = t_old + (ebar - num) / den;
+    if (fabs(tt - t_old) < tol_iter_nr) break;
+    t_old = tt;
+  }
+  return tt;
+}
+
+// mixture c_p (get_cp_dev, kernels_gpu.F90:6384-6432)
+template <int NS>
+__device__ double mixture_cp(double tt, const Thermo &th, const double *yy) {
+  int nrange[NS];
+#pragma unroll
+  for (int s = 1; s <= NS; ++s) nrange[s - 1] = locate_range<NS>(tt, th, s);
+  double cploc = 0.0;
+  for (int l = th.lo; l <= th.hi; ++l) {
+    double tpow = ipow(tt, l);
+    double cp_l = 0.0;
+#pragma unroll
+    for (int s = 1; s <= NS; ++s) cp_l = cp_l + cpc<NS>(th.cp_coeff, th, l, s, nrange[s - 1]) * yy[s - 1];
+    cploc = cploc + cp_l * tpow;
+  }
+  return cploc;
+}
+
+// species enthalpy from cp_coeff (get_species_h_from_temperature_dev, :7144-7182); with
+// the cv table the same expression is the species energy (:7104-7142)
+template <int NS>
+__device__ double species_hcoef(double tt, const double *tab, const Thermo &th, int s) {
+  int nr = locate_range<NS>(tt, th, s);
+  double hh = cpc<NS>(tab, th, th.hi + 1, s, nr);
+  for (int l = th.lo; l <= th.hi; ++l) {
+    if (l == -1) hh = hh + cpc<NS>(tab, th, l, s, nr) * log(tt);
+    else hh = hh + cpc<NS>(tab, th, l, s, nr) / (l + 1.0) * ipow(tt, l + 1);
+  }
+  return hh;
+}
+
+// ------------------------------------------------------------------ launch helpers
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int hip_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace streams
+
+// N_S dispatch: the compiled species counts.  Each kernel family instantiates its
+// templates for every entry of this list.
+#define STREAMS_NS_LIST(X) X(1) X(2) X(3) X(4) X(5) X(9) X(11) X(13) X(14)
+
+#define STREAMS_DISPATCH_NS(ns, CALL)                         \
+  switch (ns) {                                               \
+    case 1: { constexpr int NS = 1; CALL; } break;            \
+    case 2: { constexpr int NS = 2; CALL; } break;            \
+    case 3: { constexpr int NS = 3; CALL; } break;            \
+    case 4: { constexpr int NS = 4; CALL; } break;            \
+    case 5: { constexpr int NS = 5; CALL; } break;            \
+    case 9: { constexpr int NS = 9; CALL; } break;            \
+    case 11: { constexpr int NS = 11; CALL; } break;          \
+    case 13: { constexpr int NS = 13; CALL; } break;          \
+    case 14: { constexpr int NS = 14; CALL; } break;          \
+    default: return STREAMS_E_NSPECIES;                       \
+  }

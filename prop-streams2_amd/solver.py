@@ -1,0 +1,323 @@
+"""Device-resident field and the reference's RK3 orchestration over the C-ABI.
+
+``Solver`` mirrors the parts of ``equation_multideal_gpu_object`` on the hot path:
+``rk_sync`` (multideal_gpu.F90:536-621), ``compute_aux`` (:361-443), ``euler_x/y/z``
+(:948-1090), ``visflx`` (:302-359), ``compute_chemistry`` (:840-875), ``update_ghost``
+(:1165-1238), ``compute_dt`` (:1620-1657) and the Strang-split implicit chemistry of
+``advance_solution`` (:1853-1931).  Arrays keep the Fortran layouts of the reference
+(stored as C-contiguous arrays with the axes reversed: ``w[m, k, j, i]``).
+
+The numerics are executed by a *backend*: ``HipBackend`` (torch device memory + the
+``libstreams_hip.so`` entry points) is the product path.  Any object with the same
+``call/zeros/asarray/to_numpy/sync`` methods can drive the same orchestration (the tests
+use one that runs the CPU restatement, to compare both sides step by step).
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import capi
+from . import grid as gridmod
+from .mechanism import R_UNIV, Tables
+
+TOL_ITER_NR = 1.0e-12   # src/parameters.F90:49
+WRAY_RHO = (0.0, -17.0 / 60.0, -5.0 / 12.0)   # multideal.F90:5768-5773
+WRAY_GAM = (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0)
+
+
+class HipBackend:
+    """torch tensors on the GPU + libstreams_hip.so."""
+
+    name = "hip"
+
+    def __init__(self, device: str = "cuda:0"):
+        import torch
+        if not torch.cuda.is_available():
+            raise capi.StreamsError("no GPU visible: the HIP backend has no CPU fallback")
+        self.torch = torch
+        self.device = torch.device(device)
+        self.lib = capi.load()
+
+    @property
+    def stream(self):
+        return self.torch.cuda.current_stream(self.device).cuda_stream
+
+    def zeros(self, shape, dtype="f8"):
+        t = self.torch
+        dt = {"f8": t.float64, "i4": t.int32}[dtype]
+        return t.zeros(shape, dtype=dt, device=self.device)
+
+    def asarray(self, a: np.ndarray, dtype="f8"):
+        t = self.torch
+        a = np.ascontiguousarray(a, dtype={"f8": np.float64, "i4": np.int32}[dtype])
+        return t.from_numpy(a).to(self.device)
+
+    def to_numpy(self, x) -> np.ndarray:
+        return x.detach().cpu().numpy()
+
+    def copy_(self, dst, src):
+        dst.copy_(src)
+
+    def sync(self):
+        self.torch.cuda.synchronize(self.device)
+
+    def call(self, name, *args):
+        return self.lib.call(name, *args)
+
+
+@dataclasses.dataclass
+class Numerics:
+    ep_order: int = 6
+    nkeep: int = 0
+    visc_order: int = 6
+    conservative_viscous: int = 1
+    weno_scheme: int = 3
+    weno_version: int = 1
+    flux_splitting: int = 0
+    sensor_threshold: float = 2.0
+    sensor_type: int = 1
+    enable_chemistry: int = 1
+    correct_bound_ord: int = 0
+    ros_eps: float = 1e-8
+    ros_maxsteps: int = 10000
+    ros_maxtry: int = 40
+
+
+class Solver:
+    """One rank's subdomain: padded field, auxiliaries, RHS buffers and tables on a backend."""
+
+    def __init__(self, backend, tables: Tables, nx: int, ny: int, nz: int, ng: int,
+                 lengths: Sequence[float], bctags: Sequence[int] = (0, 0, 0, 0, 0, 0),
+                 numerics: Optional[Numerics] = None, rho0: float = 1.0, u0: float = 1.0,
+                 p0: Optional[float] = None, eps_sensor: Optional[float] = None):
+        self.be = backend
+        self.tb = tables
+        self.ns = tables.n_species
+        self.nv = self.ns + 4
+        self.num = numerics or Numerics()
+        self.nv_aux = 2 * self.ns + 13 + (2 * (self.ns + 1) if self.num.enable_chemistry > 0 else 0)
+        self.nx, self.ny, self.nz, self.ng = nx, ny, nz, ng
+        self.bctags = list(bctags)
+        self.periodic = np.array([int(bctags[0] == 0 and bctags[1] == 0), int(bctags[2] == 0 and bctags[3] == 0),
+                                  int(bctags[4] == 0 and bctags[5] == 0)], dtype=np.int32)
+        self.rho0, self.u0 = rho0, u0
+        self.p0 = tables.p0 if p0 is None else p0
+        self.eps_sensor = u0 * u0 if eps_sensor is None else eps_sensor
+        self.axes = [gridmod.uniform_axis(n, L, ng, bool(self.periodic[d]))
+                     for d, (n, L) in enumerate(zip((nx, ny, nz), lengths))]
+        be = backend
+        px, py, pz = nx + 2 * ng, ny + 2 * ng, nz + 2 * ng
+        self.w = be.zeros((self.nv, pz, py, px))
+        self.w_aux = be.zeros((self.nv_aux, pz, py, px))
+        self.fl = be.zeros((self.nv, nz, ny, nx))
+        self.fln = be.zeros((self.nv, nz, ny, nx))
+        self.fhat = be.zeros((self.nv, pz, py, px))
+        # tables (Fortran layouts: store transposed C arrays)
+        f = lambda a: be.asarray(np.asarray(a, dtype=np.float64).T)  # noqa: E731
+        self.t_rgas = be.asarray(tables.rgas)
+        self.t_mw = be.asarray(tables.mw)
+        self.t_mwinv = be.asarray(tables.mwinv)
+        self.t_cp = f(tables.cp_coeff)
+        self.t_cv = f(tables.cv_coeff)
+        self.t_trange = f(tables.trange)
+        self.t_visc = f(tables.visc_species)
+        self.t_lambda = f(tables.lambda_species)
+        self.t_diff = f(tables.diffbin_species)
+        self.nr = tables.nreactions if self.num.enable_chemistry > 0 else 0
+        if self.nr > 0:
+            self.t_arr_a, self.t_arr_b, self.t_arr_ea = f(tables.arr_a), f(tables.arr_b), f(tables.arr_ea)
+            self.t_fall, self.t_tb = f(tables.falloff_coeffs), f(tables.tb_eff)
+            self.t_rco, self.t_pco, self.t_kc = f(tables.r_coeffs), f(tables.p_coeffs), f(tables.kc_tab)
+            self.t_rty = be.asarray(tables.reac_ty, "i4")
+            self.t_rev = be.asarray(tables.isRev, "i4")
+        self.cd1 = f(gridmod.coeff_deriv1_table())
+        self.t_x, self.t_y, self.t_z = (be.asarray(a.coord) for a in self.axes)
+        self.dxi = [be.asarray(a.dxi) for a in self.axes]
+        self.dxi2 = [be.asarray(a.dxi2) for a in self.axes]
+        epo = gridmod.ep_ord_change(nx, ny, nz, self.bctags, self.num.ep_order, self.num.weno_scheme,
+                                    self.num.correct_bound_ord)
+        self.t_epo = be.asarray(epo.T, "i4")
+        self.t_periodic = self.periodic.copy()
+        # Bilger mixture fraction off (N_EoI = 1)
+        self.t_eoi = be.asarray(np.ones(1))
+        self.t_nainsp = be.asarray(np.zeros(self.ns, dtype=np.int32), "i4")
+        self.t_beta0 = be.asarray(np.zeros(2))
+        self.eul = (1, nx, 1, ny, 1, nz)   # bc_preproc without non-reflecting faces
+        self.force_zero_flux = [0] * 6
+        self.dt = 0.0
+        self.time = 0.0
+
+    # ------------------------------------------------------------------ data
+    def set_state(self, w_padded: np.ndarray, t_guess: np.ndarray):
+        """w_padded[m,k,j,i] conserved variables incl. ghosts; t_guess the Newton start."""
+        be = self.be
+        be.copy_(self.w, be.asarray(w_padded))
+        aux = np.zeros((self.nv_aux,) + w_padded.shape[1:])
+        aux[self.ns + 4] = t_guess
+        be.copy_(self.w_aux, be.asarray(aux))
+
+    # ------------------------------------------------------------- kernels
+    def eval_aux(self, box=None):
+        nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
+        if box is None:
+            box = (1 - ng, nx + ng, 1 - ng, ny + ng, 1 - ng, nz + ng)
+        tb = self.tb
+        self.be.call("eval_aux", self.ns, nx, ny, nz, ng, self.nv, self.nv_aux, *box, self.w, self.w_aux,
+                     self.p0, tb.t_min_tab, tb.dt_tab, R_UNIV, self.t_rgas, self.t_cv, self.t_cp, tb.nsetcv,
+                     self.t_trange, tb.indx_cp_l, tb.indx_cp_r, TOL_ITER_NR, self.be.stream, self.t_mw,
+                     self.t_mwinv, self.t_visc, self.t_lambda, self.t_diff, tb.num_t_tab, 1, self.t_eoi,
+                     self.t_nainsp, self.t_beta0, self.t_eoi)
+
+    def eval_aux2(self):
+        self.be.call("eval_aux2", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, self.num.visc_order,
+                     self.w_aux, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], self.eps_sensor,
+                     self.num.sensor_type, self.be.stream)
+
+    def compute_aux(self, central: int = 1, ghost: int = 1):
+        """compute_aux (multideal_gpu.F90:361-443)."""
+        nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
+        if central == 1 and ghost == 1:
+            self.eval_aux()
+        elif central == 1 and ghost == 0:
+            self.eval_aux((1, nx, 1, ny, 1, nz))
+        elif central == 0 and ghost == 1:
+            for box in ((1 - ng, 0, 1 - ng, ny + ng, 1 - ng, nz + ng), (nx + 1, nx + ng, 1 - ng, ny + ng, 1 - ng, nz + ng),
+                        (1 - ng, nx + ng, 1 - ng, 0, 1 - ng, nz + ng), (1 - ng, nx + ng, ny + 1, ny + ng, 1 - ng, nz + ng),
+                        (1 - ng, nx + ng, 1 - ng, ny + ng, 1 - ng, 0), (1 - ng, nx + ng, 1 - ng, ny + ng, nz + 1, nz + ng)):
+                self.eval_aux(box)
+        if ghost == 1:
+            self.eval_aux2()
+
+    def bcswap(self):
+        self.be.call("bcswap", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.t_periodic, self.be.stream)
+
+    def bcswap_var(self, m: int):
+        """Exchange one auxiliary variable (1-based index m)."""
+        self.be.call("bcswap", self.nx, self.ny, self.nz, self.ng, 1, self.w_aux[m - 1], self.t_periodic,
+                     self.be.stream)
+
+    def euler(self, d: int):
+        tb, n = self.tb, self.num
+        lmax = n.ep_order // 2
+        args_common = (self.ns, self.nv, self.nv_aux, self.nx, self.ny, self.nz, self.ng)
+        if d == 0:
+            imin, imax = self.eul[0], self.eul[1]
+            self.be.call("euler_x", *args_common, imin - 1, imax, lmax, n.nkeep, self.t_rgas, self.w_aux, self.cd1,
+                         self.dxi[0], self.fhat, self.force_zero_flux[0], self.force_zero_flux[1], n.weno_scheme,
+                         n.weno_version, n.sensor_threshold, 2 * n.weno_scheme, self.t_cv, self.t_cp, tb.nsetcv,
+                         self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.t_epo, TOL_ITER_NR, self.rho0, self.u0,
+                         n.flux_splitting, imin, imax, self.fl, self.be.stream)
+        else:
+            name = "euler_y" if d == 1 else "euler_z"
+            lo, hi = self.eul[2 * d], self.eul[2 * d + 1]
+            self.be.call(name, *args_common, lo, hi, lmax, n.nkeep, self.t_rgas, self.w_aux, self.fl, self.cd1,
+                         self.dxi[d], self.fhat, self.force_zero_flux[2 * d], self.force_zero_flux[2 * d + 1],
+                         n.weno_scheme, n.weno_version, n.sensor_threshold, 2 * n.weno_scheme, self.t_cv,
+                         self.t_cp, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.t_epo, TOL_ITER_NR,
+                         self.rho0, self.u0, n.flux_splitting, self.be.stream)
+
+    def visflx(self, mode: int):
+        tb = self.tb
+        if mode == 1:
+            for d, (name, coord) in enumerate((("visflx_x", self.t_x), ("visflx_y", self.t_y), ("visflx_z", self.t_z))):
+                if (self.nx, self.ny, self.nz)[d] > 1:
+                    self.be.call(name, self.ns, self.nx, self.ny, self.nz, self.nv, self.nv_aux, self.ng, coord,
+                                 self.w_aux, self.fl, self.fhat, tb.indx_cp_l, tb.indx_cp_r, self.t_cp, tb.nsetcv,
+                                 self.t_trange, self.t_rgas, R_UNIV, self.be.stream)
+        elif mode == 2:
+            self.be.call("visflx_reduced", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux,
+                         self.num.visc_order, self.w_aux, self.fl, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2],
+                         self.be.stream)
+        else:
+            raise NotImplementedError("Laplacian viscous form (conservative_viscous=0)")
+
+    def compute_chemistry(self):
+        tb = self.tb
+        self.be.call("compute_chemistry", self.ns, self.nx, self.ny, self.nz, self.nv, self.nv_aux, self.ng, self.nr,
+                     self.w, self.w_aux, self.fl, self.t_mw, self.t_arr_a, self.t_arr_b, self.t_arr_ea, self.t_tb,
+                     self.t_fall, self.t_rty, self.t_rev, self.t_rco, self.t_pco, self.t_kc, tb.num_t_tab,
+                     tb.t_min_tab, tb.dt_tab, R_UNIV, tb.indx_cp_l, tb.indx_cp_r, self.t_cv, tb.nsetcv,
+                     self.t_trange, TOL_ITER_NR, self.be.stream)
+
+    def rosenbrock(self, dt_chem: float):
+        tb, n = self.tb, self.num
+        self.be.call("rosenbrock", self.ns, self.nv, self.nv_aux, self.nx, self.ny, self.nz, self.ng, self.w,
+                     self.w_aux, self.fl, self.t_mw, self.nr, self.t_arr_a, self.t_arr_b, self.t_arr_ea, self.t_tb,
+                     self.t_fall, self.t_rty, self.t_rev, self.t_rco, self.t_pco, self.t_kc, tb.num_t_tab,
+                     tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l,
+                     tb.indx_cp_r, dt_chem, self.time, self.time + dt_chem, n.ros_maxsteps, n.ros_maxtry, n.ros_eps,
+                     0, 0, 0.0, 0.0, 0.0, self.dxi[0], self.dxi[1], self.dxi[2], self.be.stream)
+
+    def update_ghost(self):
+        """Physical boundary conditions of the non-periodic faces (update_ghost, do_swap=0)."""
+        for ilat, tag in enumerate(self.bctags, start=1):
+            if tag == 0:
+                continue
+            if tag == 2:
+                self.be.call("bcextr", ilat, self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.be.stream)
+            elif tag == 8:
+                self.be.call("bcwall_ad", self.ns, ilat, self.nx, self.ny, self.nz, self.ng, self.w, self.be.stream)
+            else:
+                raise NotImplementedError(f"boundary tag {tag}")
+
+    # -------------------------------------------------------------- RK step
+    def rk_substep(self, istep: int):
+        """One substep of rk_sync (multideal_gpu.F90:562-618)."""
+        n = self.num
+        rhodt = WRAY_RHO[istep] * self.dt
+        gamdt = WRAY_GAM[istep] * self.dt
+        be = self.be
+        be.call("init_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, rhodt, be.stream)
+        self.bcswap()
+        self.compute_aux()
+        jduc, jdiv = self.ns + 9, self.ns + 10
+        be.call("bcextr_var", self.nx, self.ny, self.nz, self.ng, self.w_aux[jdiv - 1], be.stream)
+        self.bcswap_var(jduc)
+        self.bcswap_var(jdiv)
+        self.euler(0)
+        if self.ny > 1:
+            self.euler(1)
+        if self.nz > 1:
+            self.euler(2)
+        if n.conservative_viscous == 1:
+            self.visflx(1)
+            self.visflx(2)
+        else:
+            self.visflx(0)
+        if n.enable_chemistry == 1:
+            self.compute_chemistry()
+        be.call("update_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, gamdt, be.stream)
+        be.call("update_field", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.fln, None, be.stream)
+        self.update_ghost()
+
+    def rk_sync(self):
+        for istep in range(3):
+            self.rk_substep(istep)
+
+    def advance(self):
+        """advance_solution (multideal_gpu.F90:1853-1931) without LES / energy deposition."""
+        if self.num.enable_chemistry == 2:
+            dt_chem = 0.5 * self.dt
+            self.rosenbrock(dt_chem)
+            self.rk_sync()
+            self.compute_aux(central=1, ghost=0)
+            self.rosenbrock(dt_chem)
+        else:
+            self.rk_sync()
+        self.time += self.dt
+
+    def compute_dt(self, cfl: float) -> float:
+        """compute_dt (multideal_gpu.F90:1620-1657)."""
+        if cfl < 0:
+            self.dt = -cfl
+            return self.dt
+        tb = self.tb
+        out = np.zeros(9)
+        self.be.call("compute_dt", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv, self.nv_aux, self.dxi[0],
+                     self.dxi[1], self.dxi[2], self.dxi2[0], self.dxi2[1], self.dxi2[2], self.w, self.w_aux, out,
+                     tb.indx_cp_l, tb.indx_cp_r, self.t_cp, tb.nsetcv, self.t_trange, None, self.be.stream)
+        self.dt = cfl / float(out.max())
+        return self.dt

@@ -1,0 +1,276 @@
+"""GPU parity: each HIP entry point against the CPU restatement on identical seeded inputs.
+
+Tolerance (north_star): RHS agreement within 1e-12 relative in the max-norm of each
+conserved variable; species and temperature after N steps within 1e-9 relative.
+Auxiliary variables are held to the same 1e-12.  Copies (halo exchange) are bit-exact.
+Shapes are deliberately non-cubic so that a transposed stride or a swapped extent fails.
+"""
+import numpy as np
+import pytest
+
+from parity_util import box_state, faces_only_mask, interior, rel_maxnorm
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+SHAPE = (18, 12, 10)   # nx, ny, nz
+NG = 3
+
+
+@pytest.fixture(scope="module")
+def hip_backend(pkg):
+    return pkg.solver.HipBackend()
+
+
+@pytest.fixture(scope="module")
+def cpu_backend(oracle):
+    import oracle_backend
+    return oracle_backend.OracleBackend(oracle)
+
+
+def make_pair(pkg, tables, hip_backend, cpu_backend, shape=SHAPE, numerics=None, seed=11, **kw):
+    w, tg, info = box_state(pkg, tables, shape, seed=seed, **kw)
+    out = []
+    for be in (hip_backend, cpu_backend):
+        s = pkg.solver.Solver(be, tables, *shape, NG, [info["length"]] * 3, numerics=numerics,
+                              rho0=info["rho0"], u0=info["u_rms"])
+        s.set_state(w, tg)
+        out.append(s)
+    return out[0], out[1], info
+
+
+def sync_aux(g, c):
+    """Give the GPU solver the CPU solver's auxiliaries (isolates one kernel)."""
+    g.be.copy_(g.w_aux, g.be.asarray(c.w_aux))
+    g.be.copy_(g.w, g.be.asarray(c.w))
+
+
+def prepare_aux(s):
+    s.compute_aux()
+    jduc, jdiv = s.ns + 9, s.ns + 10
+    s.be.call("bcextr_var", s.nx, s.ny, s.nz, s.ng, s.w_aux[jdiv - 1], s.be.stream)
+    s.bcswap_var(jduc)
+    s.bcswap_var(jdiv)
+
+
+def test_library_reports_species(pkg):
+    assert 9 in pkg.capi.species_supported()
+    assert 14 in pkg.capi.species_supported()
+
+
+def test_eval_aux(pkg, h2_tables, hip_backend, cpu_backend):
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend, perturb_y=0.3)
+    g.eval_aux()
+    c.eval_aux()
+    g.be.sync()
+    ns = h2_tables.n_species
+    written = list(range(ns)) + list(range(ns, ns + 8)) + [ns + 10, ns + 11, ns + 12] + \
+        list(range(ns + 13, 2 * ns + 13))
+    ga, ca = g.be.to_numpy(g.w_aux)[written], c.w_aux[written]
+    err = rel_maxnorm(ga, ca)
+    assert err.max() < TOL, err
+    assert np.array_equal(g.be.to_numpy(g.w), c.w)
+
+
+def test_eval_aux_subbox_and_clip(pkg, h2_tables, hip_backend, cpu_backend):
+    """A sub-box of the padded field (compute_aux central=1, ghost=0) and the rho*Y >= 0 clip."""
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend)
+    wn = c.w.copy()
+    wn[3, 5, 6, 7] = -1e-9                    # a negative partial density is clipped in w
+    for s in (g, c):
+        s.be.copy_(s.w, s.be.asarray(wn))
+        s.compute_aux(central=1, ghost=0)
+    g.be.sync()
+    assert np.array_equal(g.be.to_numpy(g.w), c.w)
+    assert c.w[3, 5, 6, 7] == 0.0
+    err = rel_maxnorm(interior(g.be.to_numpy(g.w_aux), NG)[: 2 * h2_tables.n_species + 13],
+                      interior(c.w_aux, NG)[: 2 * h2_tables.n_species + 13])
+    err[h2_tables.n_species + 8: h2_tables.n_species + 10] = 0.0   # sensor slots are not written
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("sensor_type", [0, 1])
+def test_eval_aux2(pkg, h2_tables, hip_backend, cpu_backend, sensor_type):
+    num = pkg.solver.Numerics(sensor_type=sensor_type)
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend, numerics=num)
+    c.eval_aux()
+    sync_aux(g, c)
+    g.eval_aux2()
+    c.eval_aux2()
+    ns = h2_tables.n_species
+    err = rel_maxnorm(interior(g.be.to_numpy(g.w_aux), NG)[ns + 8: ns + 10], interior(c.w_aux, NG)[ns + 8: ns + 10])
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (1, 0, 1), (0, 1, 0)])
+def test_bcswap_bit_exact(pkg, hip_backend, cpu_backend, periodic):
+    rng = np.random.default_rng(5)
+    nx, ny, nz = SHAPE
+    a = rng.standard_normal((4, nz + 2 * NG, ny + 2 * NG, nx + 2 * NG))
+    per = np.array(periodic, dtype=np.int32)
+    ga = hip_backend.asarray(a)
+    ca = a.copy()
+    hip_backend.call("bcswap", nx, ny, nz, NG, 4, ga, per, hip_backend.stream)
+    cpu_backend.call("bcswap", nx, ny, nz, NG, 4, ca, per, None)
+    assert np.array_equal(hip_backend.to_numpy(ga), ca)
+
+
+def test_bcswap_short_axis_bit_exact(pkg, hip_backend, cpu_backend):
+    """An axis shorter than ng (the 1-D tube's y and z) takes the staged path."""
+    rng = np.random.default_rng(6)
+    nx, ny, nz = 9, 1, 2
+    a = rng.standard_normal((3, nz + 2 * NG, ny + 2 * NG, nx + 2 * NG))
+    per = np.array([0, 1, 1], dtype=np.int32)
+    ga = hip_backend.asarray(a)
+    ca = a.copy()
+    hip_backend.call("bcswap", nx, ny, nz, NG, 3, ga, per, hip_backend.stream)
+    cpu_backend.call("bcswap", nx, ny, nz, NG, 3, ca, per, None)
+    assert np.array_equal(hip_backend.to_numpy(ga), ca)
+
+
+@pytest.mark.parametrize("ilat", [1, 2, 3, 4, 5, 6])
+def test_physical_bc_bit_exact(pkg, hip_backend, cpu_backend, ilat):
+    rng = np.random.default_rng(ilat)
+    nx, ny, nz = SHAPE
+    ns = 9
+    a = rng.standard_normal((ns + 4, nz + 2 * NG, ny + 2 * NG, nx + 2 * NG))
+    for name, args in (("bcextr", (ilat, nx, ny, nz, NG, ns + 4)), ("bcwall_ad", (ns, ilat, nx, ny, nz, NG))):
+        ga = hip_backend.asarray(a)
+        ca = a.copy()
+        hip_backend.call(name, *args, ga, hip_backend.stream)
+        cpu_backend.call(name, *args, ca, None)
+        assert np.array_equal(hip_backend.to_numpy(ga), ca), name
+    ga = hip_backend.asarray(a[0])
+    ca = a[0].copy()
+    hip_backend.call("bcextr_var", nx, ny, nz, NG, ga, hip_backend.stream)
+    cpu_backend.call("bcextr_var", nx, ny, nz, NG, ca, None)
+    assert np.array_equal(hip_backend.to_numpy(ga), ca)
+
+
+def _flux_case(pkg, h2_tables, hip_backend, cpu_backend, numerics, stage, shape=SHAPE, **kw):
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend, numerics=numerics, shape=shape, **kw)
+    prepare_aux(c)
+    sync_aux(g, c)
+    for s in (g, c):
+        stage(s)
+    g.be.sync()
+    return rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("nkeep", [0, 1, -1])
+def test_euler_central(pkg, h2_tables, hip_backend, cpu_backend, d, nkeep):
+    num = pkg.solver.Numerics(nkeep=nkeep, sensor_threshold=2.0)
+    err = _flux_case(pkg, h2_tables, hip_backend, cpu_backend, num, lambda s: s.euler(d))
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("splitting,version", [(0, 1), (0, 0), (1, 1)])
+def test_euler_weno(pkg, h2_tables, hip_backend, cpu_backend, d, splitting, version):
+    """Threshold 0.05 with the classic sensor puts a large share of faces on the WENO branch."""
+    num = pkg.solver.Numerics(sensor_threshold=0.05, sensor_type=0, flux_splitting=splitting,
+                              weno_version=version)
+    err = _flux_case(pkg, h2_tables, hip_backend, cpu_backend, num, lambda s: s.euler(d), mach_t=0.6)
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("d", [0, 2])
+def test_euler_boundary_order_reduction(pkg, h2_tables, hip_backend, cpu_backend, d):
+    """Non-periodic faces with correct_bound_ord: reduced KEEP order and zero-flux faces."""
+    num = pkg.solver.Numerics(correct_bound_ord=1, sensor_threshold=0.1, sensor_type=0)
+    w, tg, info = box_state(pkg, h2_tables, SHAPE, seed=3, mach_t=0.5)
+    out = []
+    for be in (hip_backend, cpu_backend):
+        s = pkg.solver.Solver(be, h2_tables, *SHAPE, NG, [info["length"]] * 3, numerics=num,
+                              bctags=(2, 8, 0, 0, 2, 2), rho0=info["rho0"], u0=info["u_rms"])
+        s.set_state(w, tg)
+        s.force_zero_flux = [1, 0, 0, 0, 0, 1]
+        out.append(s)
+    g, c = out
+    prepare_aux(c)
+    sync_aux(g, c)
+    for s in (g, c):
+        s.euler(d)
+    err = rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_visflx(pkg, h2_tables, hip_backend, cpu_backend, mode):
+    err = _flux_case(pkg, h2_tables, hip_backend, cpu_backend, None, lambda s: s.visflx(mode), perturb_y=0.3)
+    assert err.max() < TOL, err
+
+
+def test_chemistry(pkg, h2_tables, hip_backend, cpu_backend):
+    """Explicit source at a temperature where every reaction class is active."""
+    err = _flux_case(pkg, h2_tables, hip_backend, cpu_backend, None, lambda s: s.compute_chemistry(),
+                     temperature=1500.0, perturb_y=0.5)
+    assert err.max() < TOL, err
+
+
+def test_compute_dt(pkg, h2_tables, hip_backend, cpu_backend):
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend)
+    c.compute_aux()
+    sync_aux(g, c)
+    dg, dc = g.compute_dt(0.5), c.compute_dt(0.5)
+    assert abs(dg - dc) <= 1e-14 * dc
+
+
+def test_rk_step_full(pkg, h2_tables, hip_backend, cpu_backend):
+    """Three Wray substeps of the whole RHS (halo, aux, sweeps, viscous, chemistry, update)."""
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend, perturb_y=0.2, temperature=1400.0)
+    for s in (g, c):
+        s.compute_aux()
+        s.compute_dt(0.5)
+    g.dt = c.dt
+    for s in (g, c):
+        s.rk_substep(0)
+    g.be.sync()
+    err = rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+    assert err.max() < TOL, err
+    for s in (g, c):
+        s.rk_substep(1)
+        s.rk_substep(2)
+    g.be.sync()
+    ns = h2_tables.n_species
+    err = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
+    assert err.max() < 1e-13, err
+    for s in (g, c):
+        s.compute_aux(central=1, ghost=0)
+    tg = interior(g.be.to_numpy(g.w_aux), NG)[ns + 4]
+    tc = interior(c.w_aux, NG)[ns + 4]
+    assert np.abs(tg - tc).max() / tc.max() < 1e-12
+
+
+def test_n_steps_species_and_temperature(pkg, h2_tables, hip_backend, cpu_backend):
+    """N full RK3 steps: species and temperature within 1e-9 relative."""
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend, shape=(12, 10, 8), temperature=1500.0,
+                        perturb_y=0.2)
+    for s in (g, c):
+        s.compute_aux()
+        s.compute_dt(0.5)
+    g.dt = c.dt
+    for _ in range(5):
+        g.advance()
+        c.advance()
+    for s in (g, c):
+        s.compute_aux(central=1, ghost=0)
+    ns = h2_tables.n_species
+    ga, ca = interior(g.be.to_numpy(g.w_aux), NG), interior(c.w_aux, NG)
+    err = rel_maxnorm(ga[: ns], ca[: ns])
+    assert err.max() < 1e-9, err
+    assert np.abs(ga[ns + 4] - ca[ns + 4]).max() / ca[ns + 4].max() < 1e-9
+
+
+def test_rosenbrock_cells(pkg, h2_tables, hip_backend, cpu_backend):
+    """Implicit per-cell chemistry (rosenbrock_kernel) over one time step on a small box."""
+    num = pkg.solver.Numerics(enable_chemistry=2)
+    g, c, _ = make_pair(pkg, h2_tables, hip_backend, cpu_backend, shape=(6, 5, 4), numerics=num,
+                        temperature=1500.0, perturb_y=0.4)
+    for s in (g, c):
+        s.compute_aux()
+        s.rosenbrock(2e-6)
+    g.be.sync()
+    err = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
+    assert err.max() < 1e-9, err
