@@ -207,6 +207,40 @@ int streams_compute_dt(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_a
                        int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
                        const double *trange, const int *fluid_mask, void *stream);
 
+
+/* ---- fused substep stages (MI355X-first orchestration of rk_sync) ------------------- */
+/* These have no single counterpart in the reference: each folds consecutive routines of
+ * one RK substep (multideal_gpu.F90:562-618) into one pass so that face fluxes and the
+ * RK bookkeeping stay on chip.  Results agree with the routine-by-routine sequence to
+ * rounding (the order of the fl accumulation differs). */
+
+/* Convective (euler_{x,y,z}_fluxes_hybrid + update) and, with visc=1, viscous
+ * (visflx_{x,y,z}_cuf) fluxes of direction dir (1..3) on faces 0..n with one update of
+ * fl for cells 1..n; init=1 also performs init_flux_cuf (fln = -rhodt*fl_old).  metric is
+ * dcsidx/detady/dzitdz, coord is x/y/z of that direction. */
+int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, int nz, int ng, int lmax_base,
+                      int nkeep, const double *rgas, const double *w_aux, const double *coeff_deriv1,
+                      const double *metric, const double *coord, double *fhat,
+                      int force_zero_flux_min, int force_zero_flux_max, int weno_scheme,
+                      int weno_version, double sensor_threshold, const double *cv_coeff,
+                      const double *cp_coeff, int nsetcv, const double *trange, int indx_cp_l,
+                      int indx_cp_r, const int *ep_ord_change, double tol_iter_nr, double rho0,
+                      double u0, int flux_splitting, int visc, double r_univ, int init,
+                      double rhodt, double *fl, double *fln, void *stream);
+
+/* visflx_reduced_cuf (reduced=1), compute_chemistry_cuf (chem=1, with rho, Y, T taken from
+ * w_aux), update_flux_cuf and update_field_cuf in one pass over the interior. */
+int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
+                       double *w, const double *w_aux, double *fl, double *fln,
+                       const double *coeff_deriv1, const double *dcsidx, const double *detady,
+                       const double *dzitdz, int reduced, int chem, int nreactions,
+                       const double *mw, const double *arr_a, const double *arr_b,
+                       const double *arr_ea, const double *tb_eff, const double *falloff_coeffs,
+                       const int *reac_ty, const int *isrev, const double *r_coeffs,
+                       const double *p_coeffs, const double *kc_tab, int num_t_tab,
+                       double t_min_tab, double dt_tab, double r_univ, double gamdt,
+                       const int *fluid_mask, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

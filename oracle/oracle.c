@@ -23,6 +23,15 @@
 #define NSMAX 64
 #define NVMAX (NSMAX + 4)
 
+/* liboracle_omp.so (CPU-baseline timing only) splits the outermost k / column loops over
+ * threads, one z-slab per thread as the reference's CPU backend runs one rank per core;
+ * every cell is still computed by the same serial statements. */
+#ifdef ORACLE_OMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
 /* padded field (1-ng:nx+ng, 1-ng:ny+ng, 1-ng:nz+ng, 1:nv) */
 #define PX (nx + 2 * ng)
 #define PY (ny + 2 * ng)
@@ -175,10 +184,11 @@ int orc_eval_aux(int ns, int nx, int ny, int nz, int ng, int nv, int nv_aux,
                  const double *beta0, const double *coeff_eoi)
 {
     (void)nv; (void)nv_aux; (void)stream;
-    double yy[NSMAX];
+    OMP_FOR
     for (int k = kstart; k <= kend; ++k)
     for (int j = jstart; j <= jend; ++j)
     for (int i = istart; i <= iend; ++i) {
+        double yy[NSMAX];
         for (int lsp = 1; lsp <= ns; ++lsp) {
             double v = W(w, i, j, k, lsp);
             W(w, i, j, k, lsp) = v > 0.0 ? v : 0.0;   /* max(w,0) */
@@ -289,6 +299,7 @@ int orc_eval_aux2(int ns, int nx, int ny, int nz, int ng, int nv_aux, int visc_o
 {
     (void)nv_aux; (void)stream;
     int lmax = visc_order / 2;
+    OMP_FOR
     for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
@@ -958,9 +969,11 @@ static int euler_dir(int dir, int ns, int nv, int nv_aux, int nx, int ny, int nz
     int n = dir == 1 ? nx : dir == 2 ? ny : nz;
     int t1n = dir == 1 ? ny : nx;           /* transverse extents */
     int t2n = dir == 3 ? ny : nz;
+    int status = 0;
+    OMP_FOR
+    for (int t2 = 1; t2 <= t2n; ++t2) {
     double fh[NVMAX], *fhl = malloc(sizeof(double) * (size_t)(n + 2 * ng + 2) * nv);
-    if (!fhl) return -4;
-    for (int t2 = 1; t2 <= t2n; ++t2)
+    if (!fhl) { status = -4; continue; }
     for (int t1 = 1; t1 <= t1n; ++t1) {
         line_t L = {w_aux, nx, ny, nz, ng, ns, dir, 0, 0, 0};
         if (dir == 1) { L.j = t1; L.k = t2; }
@@ -981,7 +994,7 @@ static int euler_dir(int dir, int ns, int nv, int nv_aux, int nx, int ny, int nz
                 int rc = orc_weno_face(&L, f, weno_scheme, weno_version, ord, rgas, cv_coeff, cp_coeff,
                                        nsetcv, trange, indx_cp_l, indx_cp_r, tol_iter_nr, rho0, u0,
                                        flux_splitting, fh);
-                if (rc) { free(fhl); return rc; }
+                if (rc) status = rc;
             }
             for (int m = 0; m < nv; ++m) fhl[(size_t)(f + ng) * nv + m] = fh[m];
         }
@@ -993,7 +1006,8 @@ static int euler_dir(int dir, int ns, int nv, int nv_aux, int nx, int ny, int nz
         }
     }
     free(fhl);
-    return 0;
+    }
+    return status;
 }
 
 int orc_euler_x(int ns, int nv, int nv_aux, int nx, int ny, int nz, int ng,
@@ -1064,9 +1078,11 @@ static int visflx_dir(int dir, int ns, int nx, int ny, int nz, int nv, int nv_au
     int n = dir == 1 ? nx : dir == 2 ? ny : nz;
     int t1n = dir == 1 ? ny : nx;
     int t2n = dir == 3 ? ny : nz;
+    int status = 0;
+    OMP_FOR
+    for (int t2 = 1; t2 <= t2n; ++t2) {
     double *fhl = malloc(sizeof(double) * (size_t)(n + 1) * nv);
-    if (!fhl) return -4;
-    for (int t2 = 1; t2 <= t2n; ++t2)
+    if (!fhl) { status = -4; continue; }
     for (int t1 = 1; t1 <= t1n; ++t1) {
         int bi, bj, bk;
         if (dir == 1) { bi = 0; bj = t1; bk = t2; }
@@ -1137,7 +1153,8 @@ static int visflx_dir(int dir, int ns, int nx, int ny, int nz, int nv, int nv_au
         }
     }
     free(fhl);
-    return 0;
+    }
+    return status;
 }
 
 int orc_visflx_x(int ns, int nx, int ny, int nz, int nv, int nv_aux, int ng, const double *x,
@@ -1173,6 +1190,7 @@ int orc_visflx_reduced(int ns, int nx, int ny, int nz, int ng, int nv_aux, int v
 {
     (void)nv_aux; (void)stream;
     int lmax = visc_order / 2;
+    OMP_FOR
     for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
@@ -1330,10 +1348,11 @@ int orc_compute_chemistry(int ns, int nx, int ny, int nz, int nv, int nv_aux, in
                           const double *trange, double tol_iter_nr, void *stream)
 {
     (void)nv; (void)nv_aux; (void)stream;
-    double conc[NSMAX], yy[NSMAX];
+    OMP_FOR
     for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
+        double conc[NSMAX], yy[NSMAX];
         for (int lsp = 1; lsp <= ns; ++lsp) {
             double v = W(w, i, j, k, lsp);
             W(w, i, j, k, lsp) = v > 0.0 ? v : 0.0;
@@ -1597,10 +1616,11 @@ int orc_rosenbrock(int ns, int nv, int nv_aux, int nx, int ny, int nz, int ng,
     chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
                 falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
                 tol_iter_nr, cv_coeff, trange};
-    double wv[NVMAX];
+    OMP_FOR
     for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
+        double wv[NVMAX];
         for (int m = 1; m <= nv; ++m) wv[m - 1] = W(w, i, j, k, m);
         double tg = W(w_aux, i, j, k, J_T);
         int na, nr;
@@ -1629,8 +1649,9 @@ int orc_rosenbrock_batch(int ns, int nbatch, double *w, double *tg, const double
     chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
                 falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
                 tol_iter_nr, cv_coeff, trange};
-    double wv[NVMAX];
+    OMP_FOR
     for (int b = 0; b < nbatch; ++b) {
+        double wv[NVMAX];
         for (int m = 0; m < nv; ++m) wv[m] = w[(size_t)m * nbatch + b];
         int na = 0, nr = 0;
         orc_rosenbrock_cell(&C, wv, &tg[b], dttry[b], 0.0, time_end[b], maxsteps, maxtry, tol, &na, &nr);
@@ -1647,16 +1668,18 @@ int orc_rosenbrock_batch(int ns, int nbatch, double *w, double *tg, const double
 int orc_init_flux(int nx, int ny, int nz, int nv, double *fl, double *fln, double rhodt, void *stream)
 {
     (void)stream;
-    size_t n = (size_t)nx * ny * nz * nv;
-    for (size_t q = 0; q < n; ++q) { fln[q] = -rhodt * fl[q]; fl[q] = 0.0; }
+    long n = (long)nx * ny * nz * nv;
+    OMP_FOR
+    for (long q = 0; q < n; ++q) { fln[q] = -rhodt * fl[q]; fl[q] = 0.0; }
     return 0;
 }
 
 int orc_update_flux(int nx, int ny, int nz, int nv, const double *fl, double *fln, double gamdt, void *stream)
 {
     (void)stream;
-    size_t n = (size_t)nx * ny * nz * nv;
-    for (size_t q = 0; q < n; ++q) fln[q] = fln[q] - gamdt * fl[q];
+    long n = (long)nx * ny * nz * nv;
+    OMP_FOR
+    for (long q = 0; q < n; ++q) fln[q] = fln[q] - gamdt * fl[q];
     return 0;
 }
 
@@ -1664,6 +1687,7 @@ int orc_update_field(int nx, int ny, int nz, int ng, int nv, double *w, const do
                      const int *fluid_mask, void *stream)
 {
     (void)stream;
+    OMP_FOR
     for (int m = 1; m <= nv; ++m)
     for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)

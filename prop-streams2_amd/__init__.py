@@ -9,7 +9,7 @@ the batched reactor integrator and the continuous-batching reactor server.
 """
 import importlib as _importlib
 
-__all__ = ["mechanism", "capi", "grid", "cases", "solver", "reactors", "serving", "build"]
+__all__ = ["mechanism", "capi", "grid", "cases", "solver", "roofline", "reactors", "serving", "build"]
 
 
 def __getattr__(name):

@@ -42,6 +42,8 @@ _ARGS = {
     "update_flux": "iiii" "pp" "d" "s",
     "update_field": "iiiii" "ppp" "s",
     "compute_dt": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
+    "rhs_sweep": "iiiiiiiii" "ppppp" "p" "iiii" "d" "pp" "i" "p" "ii" "p" "ddd" "ii" "d" "i" "d" "pp" "s",
+    "rhs_finish": "iiiiiii" "pppppppp" "iii" "ppppppppppp" "i" "dddd" "p" "s",
 }
 
 SIGNATURES: Dict[str, str] = dict(_ARGS)

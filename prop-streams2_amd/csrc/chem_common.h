@@ -283,3 +283,93 @@ __device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_
 }
 
 }  // namespace streams
+
+namespace streams {
+
+// Species source term -omega_s*W_s accumulated into acc[] (compute_chemistry_cuf's reaction
+// loop, kernels_gpu.F90:9551-9649) with the rate constants evaluated through exp/log
+// identities: A T^b exp(-Ea/RT) = A exp(b ln T - Ea/RT), log10(x) = ln(x)/ln(10),
+// 10^x = exp(x ln 10), c^2 = c*c.  Each identity changes a result by a few units in the
+// last place only.
+template <int NS>
+__device__ inline void chemistry_source(const Kin &K, double tt, const double *conc, double *acc) {
+  const int nr = K.nr;
+  const double Rtti = 1.0 / (K.r_univ * tt);
+  const double lnT = log(tt);
+  constexpr double LN10 = 2.302585092994045684;
+  constexpr double ILN10 = 0.434294481903251827651;
+  int itt = (int)((tt - K.t_min_tab) / K.dt_tab) + 1;
+  itt = itt < 1 ? 1 : itt;
+  itt = itt > K.num_t_tab ? K.num_t_tab : itt;
+  const double ttleft = (itt - 1) * K.dt_tab + K.t_min_tab;
+  const double dtt = (tt - ttleft) / K.dt_tab;
+  for (int lr = 0; lr < nr; ++lr) {
+    const int ty = K.reac_ty[lr];
+    double kf;
+    double tb = 0.0;
+    if (ty <= 1) {
+      const double b0 = K.arr_b[lr], e0 = K.arr_ea[lr];
+      kf = K.arr_a[lr];
+      if (b0 != 0.0 || e0 != 0.0) kf = kf * exp(b0 * lnT - e0 * Rtti);
+    } else {
+      const double k0 = K.arr_a[lr] * exp(K.arr_b[lr] * lnT - K.arr_ea[lr] * Rtti);
+      const double kinf = K.arr_a[lr + nr] * exp(K.arr_b[lr + nr] * lnT - K.arr_ea[lr + nr] * Rtti);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) tb = tb + K.tb_eff[lr + nr * s] * conc[s];
+      const double rpres = k0 * tb / kinf;
+      if (ty == 2) {
+        kf = kinf * (rpres / (1.0 + rpres));
+      } else if (ty == 3) {
+        if (rpres < 1e-30) {
+          kf = k0 * tb;
+        } else {
+          const double *fc = K.falloff;
+          double fcent = (1.0 - fc[lr]) * exp(-tt / fc[lr + nr]) + fc[lr] * exp(-tt / fc[lr + 2 * nr]);
+          if (fc[lr + 3 * nr] != -3.14) fcent = fcent + exp(-fc[lr + 3 * nr] / tt);
+          fcent = log(fcent) * ILN10;
+          const double coff = -0.4 - 0.67 * fcent;
+          const double noff = 0.75 - 1.27 * fcent;
+          const double lr10 = log(rpres) * ILN10;
+          const double fo = (lr10 + coff) / (noff - 0.14 * (lr10 + coff));
+          const double foff = exp(LN10 * (fcent / (1.0 + fo * fo)));
+          kf = kinf * (rpres / (1.0 + rpres)) * foff;
+        }
+      } else {
+        const double *fc = K.falloff;
+        const double l10 = log(rpres) * ILN10;
+        const double xoff = 1.0 / (1.0 + l10 * l10);
+        const double foff = fc[lr + 3 * nr] * pow(fc[lr] * exp(-fc[lr + nr] / tt) + exp(-tt / fc[lr + 2 * nr]), xoff) *
+                            exp(fc[lr + 4 * nr] * lnT);
+        kf = kinf * (rpres / (1.0 + rpres)) * foff;
+      }
+    }
+    double kb = 0.0;
+    if (K.isrev[lr] == 1) {
+      const double *kp = K.kc_tab + (itt - 1) + (long)(K.num_t_tab + 1) * lr;
+      kb = kf / (kp[0] + (kp[1] - kp[0]) * dtt);
+    }
+    double q1 = 1.0, q2 = 1.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double rr = K.r_co[lr + nr * s], pp = K.p_co[lr + nr * s];
+      if (rr == 1.0) q1 = q1 * conc[s];
+      else if (rr == 2.0) q1 = q1 * (conc[s] * conc[s]);
+      else if (rr != 0.0) q1 = q1 * pow(conc[s], rr);
+      if (pp == 1.0) q2 = q2 * conc[s];
+      else if (pp == 2.0) q2 = q2 * (conc[s] * conc[s]);
+      else if (pp != 0.0) q2 = q2 * pow(conc[s], pp);
+    }
+    if (ty == 1) {
+      tb = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) tb = tb + K.tb_eff[lr + nr * s] * conc[s];
+    } else {
+      tb = 1.0;
+    }
+    const double q = tb * (kf * q1 - kb * q2);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = acc[s] - (K.p_co[lr + nr * s] - K.r_co[lr + nr * s]) * q * K.mw[s];
+  }
+}
+
+}  // namespace streams

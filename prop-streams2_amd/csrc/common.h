@@ -77,46 +77,96 @@ __device__ inline double ipow(double x, int n) {
   return neg ? 1.0 / r : r;
 }
 
-// Newton inversion e -> T (get_mixture_temperature_from_e_dev, kernels_gpu.F90:6532-6622)
+// Newton inversion e -> T (get_mixture_temperature_from_e_dev, kernels_gpu.F90:6532-6622).
+// The mixture sums sum_s cv(l,s,range_s)*Y_s are rebuilt only when a species changes
+// temperature range between iterations; with unchanged ranges they are the same numbers
+// the reference recomputes every iteration.
+constexpr int NCOLMAX = 9;   // NASA9: indx_cp_l = -2 .. indx_cp_r + 2 = 6
+
 template <int NS>
 __device__ double temperature_from_e(double ee, double t_start, const Thermo &th,
                                      double tol_iter_nr, const double *yy) {
   int nrange[NS];
 #pragma unroll
   for (int s = 1; s <= NS; ++s) nrange[s - 1] = locate_range<NS>(t_start, th, s);
+  double cvl[NCOLMAX];
+  double sumb = 0.0;
+  bool dirty = true;
+  const int nl = th.hi - th.lo + 1;
   double t_old = t_start, tt = t_start;
   for (int iter = 1; iter <= 10; ++iter) {
     if (th.nsetcv > 1) {
 #pragma unroll
       for (int s = 1; s <= NS; ++s) {
         int nr = nrange[s - 1];
+        const int nr0 = nr;
         if (t_old < th.trange[(s - 1) + NS * (nr - 1)]) nr = nr - 1;
         else if (t_old > th.trange[(s - 1) + NS * nr]) nr = nr + 1;
         nr = nr < 1 ? 1 : nr;
-        nrange[s - 1] = nr > th.nsetcv ? th.nsetcv : nr;
+        nr = nr > th.nsetcv ? th.nsetcv : nr;
+        dirty = dirty || (nr != nr0);
+        nrange[s - 1] = nr;
       }
     }
-    double sumb = 0.0;
+    if (dirty) {
+      sumb = 0.0;
 #pragma unroll
-    for (int s = 1; s <= NS; ++s) sumb = sumb + cpc<NS>(th.cv_coeff, th, th.hi + 1, s, nrange[s - 1]) * yy[s - 1];
-    double ebar = ee - sumb;
+      for (int s = 1; s <= NS; ++s) sumb = sumb + cpc<NS>(th.cv_coeff, th, th.hi + 1, s, nrange[s - 1]) * yy[s - 1];
+      for (int q = 0; q < nl; ++q) {
+        double c = 0.0;
+#pragma unroll
+        for (int s = 1; s <= NS; ++s) c = c + cpc<NS>(th.cv_coeff, th, th.lo + q, s, nrange[s - 1]) * yy[s - 1];
+        cvl[q] = c;
+      }
+      dirty = false;
+    }
+    const double ebar = ee - sumb;
     double den = 0.0, num = 0.0;
-    for (int l = th.lo; l <= th.hi; ++l) {
-      double t_pow = ipow(t_old, l);
-      double tden = t_pow, tnum;
-      if (l == -1) tnum = log(t_old);
-      else tnum = (t_old * t_pow) / (l + 1.0);
-      double cv_l = 0.0;
-#pragma unroll
-      for (int s = 1; s <= NS; ++s) cv_l = cv_l + cpc<NS>(th.cv_coeff, th, l, s, nrange[s - 1]) * yy[s - 1];
-      num = num + cv_l * tnum;
-      den = den + cv_l * tden;
+    for (int q = 0; q < nl; ++q) {
+      const int l = th.lo + q;
+      const double t_pow = ipow(t_old, l);
+      const double tnum = (l == -1) ? log(t_old) : (t_old * t_pow) / (l + 1.0);
+      num = num + cvl[q] * tnum;
+      den = den + cvl[q] * t_pow;
     }
     tt = t_old + (ebar - num) / den;
     if (fabs(tt - t_old) < tol_iter_nr) break;
     t_old = tt;
   }
   return tt;
+}
+
+// Copy the thermo tables into a block's LDS buffer (all threads call, then the caller
+// synchronises).  Returns a Thermo whose pointers refer to LDS when the tables fit in
+// `cap` doubles, the unchanged global view otherwise.
+template <int NS>
+__device__ Thermo thermo_to_lds(const Thermo &g, double *buf, int cap) {
+  const int ncol = g.hi + 2 - g.lo + 1;
+  const int ntab = ncol * NS * g.nsetcv;
+  const int ntr = NS * (g.nsetcv + 1);
+  const bool has_cv = g.cv_coeff != nullptr, has_cp = g.cp_coeff != nullptr;
+  const int need = (has_cv ? ntab : 0) + (has_cp ? ntab : 0) + ntr;
+  if (need > cap) return g;
+  Thermo t = g;
+  double *p = buf;
+  for (int q = threadIdx.x; q < ntr; q += blockDim.x) p[q] = g.trange[q];
+  t.trange = p;
+  p += ntr;
+  if (has_cv) {
+    for (int q = threadIdx.x; q < ntab; q += blockDim.x) p[q] = g.cv_coeff[q];
+    t.cv_coeff = p;
+    p += ntab;
+  }
+  if (has_cp) {
+    for (int q = threadIdx.x; q < ntab; q += blockDim.x) p[q] = g.cp_coeff[q];
+    t.cp_coeff = p;
+  }
+  return t;
+}
+
+template <int NS>
+__host__ __device__ constexpr int thermo_lds_doubles() {
+  return 2 * NCOLMAX * NS * 3 + NS * 4;   // cp and cv with up to 3 ranges, trange
 }
 
 // mixture c_p (get_cp_dev, kernels_gpu.F90:6384-6432)

@@ -67,6 +67,9 @@ class HipBackend:
     def call(self, name, *args):
         return self.lib.call(name, *args)
 
+    def has(self, name) -> bool:
+        return self.lib.has(name)
+
 
 @dataclasses.dataclass
 class Numerics:
@@ -155,9 +158,7 @@ class Solver:
         """w_padded[m,k,j,i] conserved variables incl. ghosts; t_guess the Newton start."""
         be = self.be
         be.copy_(self.w, be.asarray(w_padded))
-        aux = np.zeros((self.nv_aux,) + w_padded.shape[1:])
-        aux[self.ns + 4] = t_guess
-        be.copy_(self.w_aux, be.asarray(aux))
+        be.copy_(self.w_aux[self.ns + 4], be.asarray(t_guess))
 
     # ------------------------------------------------------------- kernels
     def eval_aux(self, box=None):
@@ -293,20 +294,64 @@ class Solver:
         be.call("update_field", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.fln, None, be.stream)
         self.update_ghost()
 
-    def rk_sync(self):
-        for istep in range(3):
-            self.rk_substep(istep)
+    def fused_supported(self) -> bool:
+        """The fused substep covers full-range sweeps (no non-reflecting offsets), the
+        conservative viscous form and explicit or no chemistry."""
+        return (self.eul == (1, self.nx, 1, self.ny, 1, self.nz) and self.num.conservative_viscous == 1
+                and self.num.enable_chemistry in (0, 1) and self.be.has("rhs_sweep") and self.be.has("rhs_finish"))
 
-    def advance(self):
+    def rk_substep_fused(self, istep: int):
+        """One RK substep in five passes: halo + aux, sensor, fused x/y/z sweeps (with
+        init_flux folded into the first), and the pointwise tail (reduced viscous terms,
+        chemistry, update_flux, update_field), then the physical boundaries."""
+        n, tb, be = self.num, self.tb, self.be
+        rhodt = WRAY_RHO[istep] * self.dt
+        gamdt = WRAY_GAM[istep] * self.dt
+        self.bcswap()
+        self.compute_aux()
+        jduc, jdiv = self.ns + 9, self.ns + 10
+        be.call("bcextr_var", self.nx, self.ny, self.nz, self.ng, self.w_aux[jdiv - 1], be.stream)
+        self.bcswap_var(jduc)
+        self.bcswap_var(jdiv)
+        lmax = n.ep_order // 2
+        first = True
+        for d, (nd, coord) in enumerate(zip((self.nx, self.ny, self.nz), (self.t_x, self.t_y, self.t_z))):
+            if d > 0 and nd <= 1:
+                continue
+            be.call("rhs_sweep", d + 1, self.ns, self.nv_aux, self.nx, self.ny, self.nz, self.ng, lmax, n.nkeep,
+                    self.t_rgas, self.w_aux, self.cd1, self.dxi[d], coord, self.fhat, self.force_zero_flux[2 * d],
+                    self.force_zero_flux[2 * d + 1], n.weno_scheme, n.weno_version, n.sensor_threshold, self.t_cv,
+                    self.t_cp, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.t_epo, TOL_ITER_NR,
+                    self.rho0, self.u0, n.flux_splitting, int(nd > 1), R_UNIV, int(first), rhodt, self.fl, self.fln,
+                    be.stream)
+            first = False
+        chem = int(n.enable_chemistry == 1 and self.nr > 0)
+        kin = (self.t_mw, self.t_arr_a, self.t_arr_b, self.t_arr_ea, self.t_tb, self.t_fall, self.t_rty, self.t_rev,
+               self.t_rco, self.t_pco, self.t_kc) if chem else (self.t_mw,) + (None,) * 10
+        be.call("rhs_finish", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, n.visc_order, self.w,
+                self.w_aux, self.fl, self.fln, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], 1, chem, self.nr,
+                *kin, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV, gamdt, None, be.stream)
+        self.update_ghost()
+
+    def rk_sync(self, fused: Optional[bool] = None):
+        if fused is None:
+            fused = self.fused_supported()
+        for istep in range(3):
+            if fused:
+                self.rk_substep_fused(istep)
+            else:
+                self.rk_substep(istep)
+
+    def advance(self, fused: Optional[bool] = None):
         """advance_solution (multideal_gpu.F90:1853-1931) without LES / energy deposition."""
         if self.num.enable_chemistry == 2:
             dt_chem = 0.5 * self.dt
             self.rosenbrock(dt_chem)
-            self.rk_sync()
+            self.rk_sync(fused=False)
             self.compute_aux(central=1, ghost=0)
             self.rosenbrock(dt_chem)
         else:
-            self.rk_sync()
+            self.rk_sync(fused=fused)
         self.time += self.dt
 
     def compute_dt(self, cfl: float) -> float:

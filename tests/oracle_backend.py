@@ -51,3 +51,6 @@ class OracleBackend:
 
     def call(self, name, *args):
         return self.lib.call(name, *args)
+
+    def has(self, name):
+        return self.lib.has(name)

@@ -37,5 +37,9 @@ def test_binding_table_matches_header(pkg):
 
 
 def test_oracle_exports_same_entry_points(pkg, oracle):
+    """Every entry point that restates one reference routine has its CPU twin (the fused
+    rhs_* stages are checked against sequences of those twins)."""
     for name in pkg.capi.SIGNATURES:
+        if name.startswith("rhs_"):
+            continue
         assert oracle.has(name), name

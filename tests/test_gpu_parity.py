@@ -274,3 +274,55 @@ def test_rosenbrock_cells(pkg, h2_tables, hip_backend, cpu_backend):
     g.be.sync()
     err = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
     assert err.max() < 1e-9, err
+
+
+# ---------------------------------------------------------------- fused substep stages
+def _fused_vs_sequence(pkg, tables, hip_backend, cpu_backend, shape=SHAPE, numerics=None, bctags=(0,) * 6,
+                       steps=1, **kw):
+    w, tg, info = box_state(pkg, tables, shape, seed=21, **kw)
+    out = []
+    for be in (hip_backend, cpu_backend):
+        s = pkg.solver.Solver(be, tables, *shape, NG, [info["length"]] * 3, numerics=numerics, bctags=bctags,
+                              rho0=info["rho0"], u0=info["u_rms"])
+        s.set_state(w, tg)
+        s.compute_aux()
+        out.append(s)
+    g, c = out
+    c.compute_dt(0.5)
+    g.dt = c.dt
+    assert g.fused_supported()
+    g.rk_substep_fused(0)
+    c.rk_substep(0)
+    g.be.sync()
+    err_fl = rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+    for _ in range(steps):
+        for q in (1, 2):
+            g.rk_substep_fused(q)
+            c.rk_substep(q)
+    g.be.sync()
+    err_w = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
+    return err_fl, err_w
+
+
+def test_fused_substep_periodic(pkg, h2_tables, hip_backend, cpu_backend):
+    err_fl, err_w = _fused_vs_sequence(pkg, h2_tables, hip_backend, cpu_backend, temperature=1500.0,
+                                       perturb_y=0.3)
+    assert err_fl.max() < TOL, err_fl
+    assert err_w.max() < 1e-13, err_w
+
+
+def test_fused_substep_weno_and_walls(pkg, h2_tables, hip_backend, cpu_backend):
+    num = pkg.solver.Numerics(sensor_threshold=0.05, sensor_type=0, correct_bound_ord=1, nkeep=1)
+    err_fl, err_w = _fused_vs_sequence(pkg, h2_tables, hip_backend, cpu_backend, numerics=num,
+                                       bctags=(8, 2, 0, 0, 2, 2), mach_t=0.6, perturb_y=0.2)
+    assert err_fl.max() < TOL, err_fl
+    assert err_w.max() < 1e-13, err_w
+
+
+def test_fused_substep_tube_like(pkg, h2_tables, hip_backend, cpu_backend):
+    """nx x 1 x 1 with walls: the fused path skips the degenerate y and z sweeps as rk_sync does."""
+    num = pkg.solver.Numerics(sensor_threshold=0.2, sensor_type=0)
+    err_fl, err_w = _fused_vs_sequence(pkg, h2_tables, hip_backend, cpu_backend, shape=(40, 1, 1), numerics=num,
+                                       bctags=(8, 2, 0, 0, 0, 0), mach_t=0.5)
+    assert err_fl.max() < TOL, err_fl
+    assert err_w.max() < 1e-13, err_w
