@@ -101,6 +101,22 @@ def cpu_baseline(pkg, tables, n_sample: int, steps: int):
                       f"median step; oracle/oracle.c FP64 restatement, OpenMP over z-slabs"}
 
 
+def copy_probe(gib: float):
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"copy_bytes_each_way": 8 * n, "ms": ms, "GBps_read_plus_write": 16 * n / ms / 1e6}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,11 +126,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=80)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many GiB (5 times): the known-size "
+                         "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU's subdomain; --gpus {args.gpus} is not supported in this build "
-              f"(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU's subdomain; --gpus {args.gpus} is not supported "
+              f"(one process drives one GPU; the multi-GPU decomposition is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     pkg = _pkg()
     import torch

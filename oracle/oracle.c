@@ -1507,6 +1507,17 @@ static void lubksb(int n, const double *a, const int *indx, double *b)
 #undef A
 }
 
+/* Solve a x = b with ludcmp + lubksb (kernels_gpu.F90:9395-9501): a is n x n column-major
+ * and is overwritten by its LU factors, b by x.  Exported for the brute-force checks. */
+int orc_lu_solve(int n, double *a, double *b)
+{
+    int indx[NVMAX];
+    if (n < 1 || n > NVMAX) return -2;
+    ludcmp(n, a, indx);
+    lubksb(n, a, indx, b);
+    return 0;
+}
+
 /* Integrate one cell's state wv[nv] from time_start to time_end (rosenbrock_kernel body).
  * Returns accepted steps in *nacc and rejected tries in *nrej. */
 int orc_rosenbrock_cell(const chem_t *C, double *wv, double *tguess, double dttry, double time_start,

@@ -1,0 +1,462 @@
+"""Pins of the CPU oracle (oracle/oracle.c) against data and mathematics that do not come
+from the oracle itself:
+
+* golden outputs of the reference's own examples: the Keromnes-2013 constant-volume batch
+  reactor against Cantera (examples/batch_reactor/OUTPUT/cantera_results.dat, compared
+  with the tolerances of examples/batch_reactor/OUTPUT/result_comp.py:7-9) and against the
+  reference's own solver output for three syngas mechanisms
+  (examples/syngas_mech_eval/OUTPUT/output_*.dat, ignition delay by the 1.05 T0 rule of
+  examples/syngas_mech_eval/OUTPUT/mech_comp.py);
+* textbook property data (JANAF thermochemistry, NIST transport properties at 300 K);
+* chemical equilibrium from Gibbs-energy minimisation (element potentials), a different
+  algorithm from the K_c tables the kinetics use;
+* closed forms: exactness of the central-difference coefficients on polynomials, the
+  convergence order of the convective (KEEP) and staggered viscous fluxes against
+  analytic derivatives, Gaussian elimination against numpy;
+* invariants: mass and element conservation of the chemical source, telescoping flux sums
+  on a periodic box, mirror symmetry, a uniform state has no right-hand side.
+"""
+import numpy as np
+import pytest
+
+import aux_util
+import oracle_backend
+import reactor_util
+
+GOLD = oracle_backend.ROOT / "tests" / "golden"
+R_UNIV = reactor_util.R_UNIV
+
+# examples/batch_reactor/multideal.ini:7-23 (T0, p0, mass fractions by species name)
+BATCH_Y0 = dict(H2=0.01495, O2=0.1149, H2O=0.00375, N2=0.58285, AR=0.00635, CO=0.27715)
+
+
+@pytest.fixture(scope="module")
+def be():
+    return oracle_backend.OracleBackend()
+
+
+def _tables(pkg, name, **kw):
+    return pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / name)).tables(p0=101325.0, **kw)
+
+
+def _batch_history(pkg, be, mech):
+    tb = _tables(pkg, mech)
+    y = np.array([BATCH_Y0.get(n.upper(), 0.0) for n in tb.names])
+    R = reactor_util.ReactorBatch(be.lib, tb, tol=1e-8)
+    w, tg = R.state(1000.0, 101325.0, y)
+    temps = np.empty(1000)
+    for row in range(1000):            # output every 2e-6 s, dt = 2e-7 s (cfl = -2e-7)
+        for _ in range(10):
+            R.advance(w, tg, 2e-7)
+        temps[row] = R.temperature(w, tg)[0]
+    return tb, w, temps
+
+
+def test_batch_reactor_against_cantera(pkg, be):
+    """Thermo, kinetics (incl. Troe fall-off and third bodies), K_c tables and the
+    Rosenbrock integrator together, over ignition of the Keromnes-2013 syngas reactor."""
+    gold = np.loadtxt(GOLD / "batch_reactor_cantera.dat")
+    tb, w, temps = _batch_history(pkg, be, "keromnes2013_14sp.yaml")
+    rho = w[:tb.n_species, 0].sum()
+    assert abs(rho - gold[0, 1]) < 1e-5
+    assert np.abs(temps - gold[:, 2]).max() < 1.0
+    assert temps[-1] > 2500.0                         # ignited and burnt out
+
+
+@pytest.mark.parametrize("mech,golden", [("keromnes2013_14sp.yaml", "syngas_keromnes2013_streams.dat"),
+                                         ("davis2005_13sp.yaml", "syngas_davis2005_streams.dat"),
+                                         ("varga2016_14sp.yaml", "syngas_varga2016_streams.dat")])
+def test_syngas_reactor_against_reference_output(pkg, be, mech, golden):
+    ref = np.loadtxt(GOLD / golden)
+    _, _, temps = _batch_history(pkg, be, mech)
+    assert np.abs(temps - ref[:, 2]).max() < 1.0
+
+    def tau(t):
+        return ref[np.argmax(t > 1.05 * t[0]), 0]
+    assert tau(temps) == pytest.approx(tau(ref[:, 2]), rel=0.02)
+
+
+# JANAF: c_p(1000 K) [J/mol/K], H(1000 K) - H(298.15 K) [kJ/mol], dfH(298.15 K) [kJ/mol]
+JANAF = {"N2": (32.697, 21.463, 0.0), "O2": (34.870, 22.707, 0.0), "H2": (30.205, 20.680, 0.0),
+         "H2O": (41.268, 26.000, -241.826), "CO": (33.183, 21.690, -110.527),
+         "CO2": (54.308, 33.397, -393.522), "AR": (20.786, 14.589, 0.0)}
+
+
+def test_thermo_against_janaf(pkg, be):
+    """T from the mixture energy (Newton on the NASA polynomials) and c_p (through the
+    sound speed) for pure species at 1000 K, the energy taken from JANAF tables."""
+    tb = _tables(pkg, "keromnes2013_14sp.yaml", transport=True, kinetics=False)
+    ns = tb.n_species
+    idx = {n.upper(): i for i, n in enumerate(tb.names)}
+    sp = list(JANAF)
+    y = np.zeros((ns, len(sp)))
+    for b, n in enumerate(sp):
+        y[idx[n], b] = 1.0
+    w = aux_util.quiescent(tb, 1000.0, 101325.0, y)
+    for b, n in enumerate(sp):
+        mw = tb.mw[idx[n]]
+        e = (JANAF[n][2] + JANAF[n][1]) * 1e6 / mw - R_UNIV / mw * 1000.0
+        w[ns + 3, b] = w[:ns, b].sum() * e
+    wa = aux_util.point_aux(pkg, be, tb, w, t_guess=900.0)
+    for b, n in enumerate(sp):
+        tt, c = wa[ns + 4, b], wa[ns + 6, b]
+        rs = R_UNIV / tb.mw[idx[n]]
+        gam = c * c / (rs * tt)
+        cp_molar = gam * rs / (gam - 1.0) * tb.mw[idx[n]] / 1000.0
+        assert abs(tt - 1000.0) < 2.5, n
+        assert cp_molar == pytest.approx(JANAF[n][0], rel=6e-3), n
+
+
+def test_energy_inversion_round_trip(pkg, be):
+    """e(T(e)) = e: the oracle's Newton temperature against a bracketing root of the host
+    NASA energy polynomial, across the range break, for random mixtures."""
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    rng = np.random.default_rng(5)
+    npts = 24
+    y = rng.dirichlet(np.ones(ns), size=npts).T
+    temps = np.exp(rng.uniform(np.log(300.0), np.log(3000.0), npts))
+    w = aux_util.quiescent(tb, temps, 101325.0, y)
+    wa = aux_util.point_aux(pkg, be, tb, w, t_guess=1500.0)
+    for b in range(npts):
+        rho = w[:ns, b].sum()
+        t_root = reactor_util.t_from_e(tb, w[ns + 3, b] / rho, y[:, b])
+        assert wa[ns + 4, b] == pytest.approx(t_root, rel=1e-11)
+        assert wa[ns + 4, b] == pytest.approx(temps[b], rel=1e-10)
+
+
+# NIST (webbook) properties at 300 K, 1 atm: viscosity [Pa s], conductivity [W/m/K]
+NIST_300K = {"N2": (17.89e-6, 25.97e-3), "O2": (20.65e-6, 26.49e-3), "H2": (8.96e-6, 186.9e-3),
+             "AR": (22.73e-6, 17.72e-3), "CO": (17.84e-6, 25.0e-3)}
+# binary diffusion coefficients at 300 K, 1 atm [m^2/s] (Marrero & Mason 1972 correlations)
+BINARY_300K = {("H2", "N2"): 7.79e-5, ("O2", "N2"): 2.06e-5, ("H2", "AR"): 8.3e-5, ("CO2", "N2"): 1.65e-5}
+
+
+def test_transport_against_property_data(pkg, be):
+    """Pure-species limits of the Wilke and Mathur mixture rules and the trace limit of
+    the Bird mixture diffusivity reproduce measured properties (kinetic-theory tables)."""
+    tb = _tables(pkg, "keromnes2013_14sp.yaml", kinetics=False)
+    ns = tb.n_species
+    idx = {n.upper(): i for i, n in enumerate(tb.names)}
+    sp = list(NIST_300K)
+    y = np.zeros((ns, len(sp)))
+    for b, n in enumerate(sp):
+        y[idx[n], b] = 1.0
+    wa = aux_util.point_aux(pkg, be, tb, aux_util.quiescent(tb, 300.0, 101325.0, y), 300.0)
+    for b, n in enumerate(sp):
+        assert wa[ns + 7, b] == pytest.approx(NIST_300K[n][0], rel=0.03), n
+        assert wa[ns + 12, b] == pytest.approx(NIST_300K[n][1], rel=0.03), n
+    pairs = list(BINARY_300K)
+    y = np.zeros((ns, len(pairs)))
+    for b, (a, c) in enumerate(pairs):
+        y[idx[a], b] = 1e-7
+        y[idx[c], b] = 1.0 - 1e-7
+    w = aux_util.quiescent(tb, 300.0, 101325.0, y)
+    wa = aux_util.point_aux(pkg, be, tb, w, 300.0)
+    for b, (a, c) in enumerate(pairs):
+        d = wa[ns + 13 + idx[a], b] / w[:ns, b].sum()
+        assert d == pytest.approx(BINARY_300K[(a, c)], rel=0.06), (a, c)
+
+
+def _equilibrium_moles(g_rt, a_el, b_el, p_over_p0):
+    """Ideal-gas equilibrium at fixed T, p by element potentials: n_k = N exp(lambda.a_k -
+    g_k/RT - ln p/p0) with A n = b.  For fixed N the element potentials minimise the convex
+    dual sum_k n_k(lambda) - lambda.b (damped Newton); N is then updated to sum_k n_k."""
+    ne = a_el.shape[0]
+    lam = np.zeros(ne)
+    n_tot = b_el.sum()
+    for _ in range(60):
+        c = g_rt + np.log(p_over_p0) - np.log(n_tot)
+
+        def dual(lm):
+            n = np.exp(np.clip(a_el.T @ lm - c, -700.0, 700.0))
+            return n.sum() - lm @ b_el, n
+        for _ in range(400):
+            f0, n = dual(lam)
+            grad = a_el @ n - b_el
+            if np.abs(grad).max() <= 1e-15 * np.abs(b_el).max():
+                break
+            hess = (a_el * n) @ a_el.T
+            step = np.linalg.solve(hess + 1e-300 * np.eye(ne), grad)
+            t = 1.0
+            while t > 1e-12:
+                f1, _ = dual(lam - t * step)
+                if f1 <= f0 - 1e-4 * t * (grad @ step):
+                    break
+                t *= 0.5
+            lam = lam - t * step
+        new_tot = n.sum()
+        if abs(new_tot - n_tot) <= 1e-15 * n_tot:
+            break
+        n_tot = new_tot
+    return n / n.sum()
+
+
+def test_kinetics_relax_to_gibbs_equilibrium(pkg, be):
+    """A constant-volume H2-air reactor integrated to rest reaches the composition that
+    minimises the Gibbs energy at its final T and p (K_c tables, reverse rates, units)."""
+    mech = pkg.cases.h2_air_mechanism()
+    tb = mech.tables()
+    ns = tb.n_species
+    names = [n.upper() for n in tb.names]
+    x0 = np.zeros(ns)
+    x0[names.index("H2")], x0[names.index("O2")], x0[names.index("N2")] = 2.0, 1.0, 3.76
+    y0 = x0 * tb.mw / (x0 * tb.mw).sum()
+    R = reactor_util.ReactorBatch(be.lib, tb, tol=1e-10, maxsteps=200000)
+    w, tg = R.state(1500.0, 101325.0, y0)
+    R.integrate(w, tg, 1e-9, 0.05)
+    rho = w[:ns, 0].sum()
+    y = w[:ns, 0] / rho
+    tt = R.temperature(w, tg)[0]
+    pp = rho * tb.rmix(y) * tt
+    x = y / tb.mw / (y / tb.mw).sum()
+    elements = sorted({e for s in mech.species for e in s.composition})
+    a_el = np.array([[s.composition.get(e, 0.0) for s in mech.species] for e in elements])
+    g_rt = np.array([np.subtract(*mech.h_RT_s_R(k, np.array([tt])))[0] for k in range(ns)])
+    b_el = a_el @ (y0 / tb.mw)
+    x_eq = _equilibrium_moles(g_rt, a_el, b_el, pp / 101325.0)
+    assert tt > 2500.0
+    big = x_eq > 1e-6
+    np.testing.assert_allclose(x[big], x_eq[big], rtol=2e-5)
+    assert np.abs(x - x_eq).max() < 1e-8 + 2e-5 * x_eq.max()
+
+
+def test_chemical_source_conserves_mass_and_elements(pkg, be):
+    tb = pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / "keromnes2013_14sp.yaml")).tables(p0=101325.0)
+    mech = pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / "keromnes2013_14sp.yaml"))
+    ns = tb.n_species
+    rng = np.random.default_rng(17)
+    npts = 40
+    y = rng.dirichlet(0.3 * np.ones(ns), size=npts).T
+    temps = rng.uniform(900.0, 2600.0, npts)
+    w = aux_util.quiescent(tb, temps, rng.uniform(0.5, 20.0, npts) * 101325.0, y)
+    s = pkg.solver.Solver(be, tb, npts, 1, 1, 3, [1.0, 1.0, 1.0])
+    wp = np.empty((ns + 4, 7, 7, npts + 6))
+    wp[:] = np.concatenate([np.repeat(w[:, :1], 3, 1), w, np.repeat(w[:, -1:], 3, 1)], 1)[:, None, None, :]
+    s.set_state(wp, np.full(wp.shape[1:], 1500.0))
+    s.eval_aux()
+    s.fl[:] = 0.0
+    s.compute_chemistry()
+    fl = s.fl[:, 0, 0, :]
+    scale = np.abs(fl[:ns]).max(axis=0)
+    assert (scale > 0).all()
+    assert np.abs(fl[:ns].sum(axis=0)).max() <= 1e-12 * scale.max()
+    assert np.abs(fl[ns:]).max() == 0.0                    # no momentum/energy source
+    elements = sorted({e for sp in mech.species for e in sp.composition})
+    a_el = np.array([[sp.composition.get(e, 0.0) for sp in mech.species] for e in elements])
+    el_rate = a_el @ (fl[:ns] / tb.mw[:, None])
+    assert np.abs(el_rate).max() <= 1e-12 * (scale / tb.mw.min()).max()
+
+
+def test_lu_against_gaussian_elimination(oracle):
+    """ludcmp + lubksb on random systems, including ones that need row exchanges and the
+    Rosenbrock shape I/(gamma h) - J."""
+    import ctypes
+    fn = oracle.lib.orc_lu_solve
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 5, 13, 18):
+        for case in range(4):
+            a = rng.standard_normal((n, n))
+            if case == 1:
+                a[np.arange(n), np.arange(n)] = 1e-14     # forces pivoting
+            if case == 2:
+                a = np.eye(n) / (0.5 * 1e-7) - 1e4 * rng.standard_normal((n, n))
+            if case == 3:
+                a = a[rng.permutation(n)] * np.exp(rng.uniform(-8, 8, n))[:, None]
+            b = rng.standard_normal(n)
+            af = np.asfortranarray(a).copy(order="F")
+            buf = np.ascontiguousarray(af.T)               # column-major memory
+            x = b.copy()
+            assert fn(n, buf.ctypes.data, x.ctypes.data) == 0
+            ref = np.linalg.solve(a, b)
+            cond = np.linalg.cond(a)
+            assert np.abs(x - ref).max() <= 1e-14 * cond * np.abs(ref).max() + 1e-300, (n, case)
+
+
+@pytest.mark.parametrize("lmax", [1, 2, 3, 4])
+def test_central_difference_coefficients_are_exact_on_polynomials(pkg, lmax):
+    cd = pkg.grid.coeff_deriv1_table()
+    c = np.array([cd[l, lmax - 1] for l in range(lmax)])
+    for p in range(0, 2 * lmax + 1):
+        x0 = 0.37
+        d = sum(c[l - 1] * ((x0 + l) ** p - (x0 - l) ** p) for l in range(1, lmax + 1))
+        exact = p * x0 ** (p - 1) if p > 0 else 0.0
+        assert d == pytest.approx(exact, rel=1e-12, abs=1e-12), p
+    p = 2 * lmax + 1                                       # first polynomial not differentiated exactly
+    d = sum(c[l - 1] * (l ** p - (-l) ** p) for l in range(1, lmax + 1))
+    assert abs(d) > 1e-3
+
+
+# ----------------------------------------------------------------- 1-D convergence orders
+def _line_solver(pkg, be, tb, n, length, bctags=(0, 0, 0, 0, 0, 0), numerics=None):
+    return pkg.solver.Solver(be, tb, n, 1, 1, 3, [length, 1.0, 1.0], bctags=bctags,
+                             numerics=numerics or pkg.solver.Numerics(enable_chemistry=0))
+
+
+def _line_state(tb, x, temperature, pressure, y, u=0.0, v=0.0):
+    """Padded (nv, 7, 7, n+6) state from per-point profiles along x (periodic ghosts)."""
+    ns = tb.n_species
+    n = x.size
+    w = np.zeros((ns + 4, n))
+    for i in range(n):
+        yi = y[:, i]
+        rho = pressure[i] / (tb.rmix(yi) * temperature[i])
+        w[:ns, i] = rho * yi
+        w[ns, i] = rho * u[i]
+        w[ns + 1, i] = rho * v[i]
+        w[ns + 3, i] = rho * (tb.e_from_T(temperature[i], yi) + 0.5 * (u[i] ** 2 + v[i] ** 2))
+    wp = np.pad(w, [(0, 0), (3, 3)], mode="wrap")
+    return np.broadcast_to(wp[:, None, None, :], (ns + 4, 7, 7, n + 6)).copy()
+
+
+def _air(tb):
+    names = [n.upper() for n in tb.names]
+    y = np.zeros(tb.n_species)
+    y[names.index("O2")], y[names.index("N2")] = 0.233, 0.767
+    return y, names
+
+
+def test_convective_flux_is_sixth_order(pkg, be):
+    """Advection of a composition wave at uniform u, p, T: the KEEP divergence (ep_order 6)
+    converges to d(rho Y_s u)/dx, d(rho u^2 + p)/dx, d(u(rho E + p))/dx at order 6."""
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    y_air, names = _air(tb)
+    ih2 = names.index("H2")
+    length, u0, t0, p0 = 1.0, 120.0, 1000.0, 101325.0
+
+    def prim(x):
+        y = np.repeat(y_air[:, None], x.size, 1).astype(complex)
+        y[ih2] = 0.02 + 0.015 * np.sin(2 * np.pi * x / length)
+        y[:ns] = y[:ns] / y[:ns].sum(axis=0)
+        return y
+
+    def fluxes(x):
+        """analytic fluxes (complex-step friendly)"""
+        y = prim(x)
+        rmix = (tb.rgas[:, None] * y).sum(axis=0)
+        rho = p0 / (rmix * t0)
+        e = sum(tb.e_from_T(t0, np.eye(ns)[k]) * y[k] for k in range(ns))   # e is linear in Y
+        f = [rho * y[k] * u0 for k in range(ns)]
+        f += [rho * u0 * u0 + p0, 0 * x, 0 * x, u0 * (rho * (e + 0.5 * u0 * u0) + p0)]
+        return np.array(f)
+
+    errs = []
+    for n in (16, 32, 64):
+        x = (np.arange(n) + 0.5) * length / n
+        y = prim(x).real
+        wp = _line_state(tb, x, np.full(n, t0), np.full(n, p0), y, u=np.full(n, u0), v=np.zeros(n))
+        s = _line_solver(pkg, be, tb, n, length)
+        s.set_state(wp, np.full(wp.shape[1:], t0))
+        s.compute_aux()
+        s.fl[:] = 0.0
+        s.euler(0)
+        exact = fluxes(x + 1e-30j).imag / 1e-30
+        got = s.fl[:, 0, 0, :]
+        scale = np.abs(exact).max(axis=1)
+        keep = scale > 0
+        errs.append((np.abs(got - exact).max(axis=1)[keep] / scale[keep]))
+    rates = np.log2(np.array(errs[0]) / np.array(errs[1])), np.log2(np.array(errs[1]) / np.array(errs[2]))
+    assert (rates[1] > 5.5).all(), rates
+    assert (errs[2] < 5e-6).all()
+
+
+def test_viscous_fluxes_are_second_order(pkg, be):
+    """Staggered viscous flux of a shear wave, a trace-species wave and a temperature wave:
+    fl = -d/dx(mu dv/dx), -d/dx(rho D dY/dx), -d/dx(k dT/dx) at second order."""
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    y_air, names = _air(tb)
+    ih2 = names.index("H2")
+    length, t0, p0 = 1e-3, 800.0, 101325.0
+    kw = 2 * np.pi / length
+    res = {"shear": [], "trace": [], "heat": []}
+    for n in (32, 64, 128):
+        x = (np.arange(n) + 0.5) * length / n
+        ones = np.ones(n)
+        for case in res:
+            y = np.repeat(y_air[:, None], n, 1)
+            v = np.zeros(n)
+            tt = t0 * ones
+            if case == "shear":
+                v = 2.0 * np.sin(kw * x)
+            elif case == "trace":
+                y[ih2] = 1e-6 * (1.0 + 0.5 * np.sin(kw * x))
+                y = y / y.sum(axis=0)
+            else:
+                tt = t0 + 0.002 * np.sin(kw * x)
+            wp = _line_state(tb, x, tt, p0 * ones, y, u=np.zeros(n), v=v)
+            s = _line_solver(pkg, be, tb, n, length)
+            s.set_state(wp, np.full(wp.shape[1:], t0))
+            s.compute_aux()
+            s.fl[:] = 0.0
+            s.visflx(1)
+            fl = s.fl[:, 0, 0, :]
+            wa = s.w_aux[:, 3, 3, 3:-3]
+            if case == "shear":
+                mu = wa[ns + 7].mean()
+                exact = mu * 2.0 * kw * kw * np.sin(kw * x)          # -d/dx(mu dv/dx)
+                got = fl[ns + 1]
+            elif case == "trace":
+                rhod = wa[ns + 13 + ih2].mean()
+                exact = rhod * 0.5e-6 * kw * kw * np.sin(kw * x)
+                got = fl[ih2]
+            else:
+                kc = wa[ns + 12].mean()
+                exact = kc * 0.002 * kw * kw * np.sin(kw * x)
+                got = fl[ns + 3]
+            res[case].append(np.abs(got - exact).max() / np.abs(exact).max())
+    for case, e in res.items():
+        assert np.log2(e[1] / e[2]) > 1.8, (case, e)
+        assert e[2] < 2e-3, (case, e)
+
+
+# ------------------------------------------------------------------- 3-D invariants
+def _rhs_box(pkg, be, tb, wp, shape, lengths, numerics):
+    nx, ny, nz = shape
+    s = pkg.solver.Solver(be, tb, nx, ny, nz, 3, lengths, numerics=numerics)
+    s.set_state(wp, np.full(wp.shape[1:], 1200.0))
+    s.compute_aux()
+    s.fl[:] = 0.0
+    for d in range(3):
+        s.euler(d)
+    s.visflx(1)
+    return s.fl.copy()
+
+
+def test_periodic_flux_sums_vanish_and_mirror_symmetry(pkg, be):
+    from parity_util import box_state
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    shape = (12, 10, 8)
+    wp, _, info = box_state(pkg, tb, shape, seed=23, perturb_y=0.05)
+    L = [info["length"] * shape[d] / max(shape) for d in range(3)]
+    num = pkg.solver.Numerics(enable_chemistry=0)
+    fl = _rhs_box(pkg, be, tb, wp, shape, L, num)
+    # flux form on a uniform periodic grid: sum over cells of each divergence is zero
+    tot = fl.reshape(ns + 4, -1).sum(axis=1)
+    mag = np.abs(fl).reshape(ns + 4, -1).sum(axis=1)
+    assert (np.abs(tot) <= 1e-11 * mag).all(), np.abs(tot) / mag
+    # mirror x -> -x: the divergence mirrors, its x-momentum component changes sign
+    wm = wp[:, :, :, ::-1].copy()
+    wm[ns] = -wm[ns]
+    flm = _rhs_box(pkg, be, tb, wm, shape, L, num)
+    ref = fl[:, :, :, ::-1].copy()           # scalars mirror, the x-momentum flips sign
+    ref[ns] = -ref[ns]
+    scale = np.abs(fl).reshape(ns + 4, -1).max(axis=1)[:, None, None, None]
+    assert (np.abs(flm - ref) <= 1e-10 * scale).all()
+
+
+def test_uniform_state_has_no_rhs(pkg, be):
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    y_air, _ = _air(tb)
+    n = 8
+    wp1 = _line_state(tb, np.arange(n), np.full(n, 1100.0), np.full(n, 2e5),
+                      np.repeat(y_air[:, None], n, 1), u=np.full(n, 30.0), v=np.full(n, -12.0))
+    wp = np.broadcast_to(wp1[:, 3:4, 3:4, 3:4], (ns + 4, 14, 14, 14)).copy()
+    num = pkg.solver.Numerics(enable_chemistry=0)
+    fl = _rhs_box(pkg, be, tb, wp, (8, 8, 8), [1e-3] * 3, num)
+    assert np.abs(fl).max() == 0.0
