@@ -156,6 +156,7 @@ def main():
     s.compute_dt(0.5)
     torch.cuda.synchronize()
     print(f"[bench] dt = {s.dt:.3e} s; warmup {args.warmup} step(s)", file=sys.stderr, flush=True)
+    tbe.records = {}                       # profile the warm-up steps only, not the set-up
     for _ in range(args.warmup):
         s.rk_sync()
     torch.cuda.synchronize()

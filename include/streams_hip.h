@@ -228,18 +228,30 @@ int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, int nz, int 
                       double u0, int flux_splitting, int visc, double r_univ, int init,
                       double rhodt, double *fl, double *fln, void *stream);
 
+/* Kinetics records for the fused chemistry: packs the reference's kinetics tables (HOST
+ * arrays in their Fortran layouts, as read by multideal.F90:2763-2926: arr_a/b/ea(nr,2),
+ * tb_eff/r_coeffs/p_coeffs(nr,N_S), falloff_coeffs(nr,5), reac_ty/isRev(nr)) and the molar
+ * masses into nreactions records of 64 doubles each, the layout documented in
+ * csrc/kinpack.h.  Returns the number of doubles of the pack (pass out = NULL to query),
+ * STREAMS_E_DIMS if cap is too small, STREAMS_E_OPTION if a reaction has more than 4
+ * reactant or product species or 12 species with a net change, STREAMS_E_NSPECIES for
+ * N_S > 16.  The caller copies the pack to the device once, as the reference uploads its
+ * tables once (multideal_gpu.F90:1363-1424). */
+int streams_kinetics_pack(int n_s, int nreactions, const double *arr_a, const double *arr_b,
+                          const double *arr_ea, const double *tb_eff, const double *falloff_coeffs,
+                          const int *reac_ty, const int *isrev, const double *r_coeffs,
+                          const double *p_coeffs, const double *mw, double *out, int cap);
+
 /* visflx_reduced_cuf (reduced=1), compute_chemistry_cuf (chem=1, with rho, Y, T taken from
- * w_aux), update_flux_cuf and update_field_cuf in one pass over the interior. */
+ * w_aux; kinetics from the device copy of streams_kinetics_pack's records), update_flux_cuf
+ * and update_field_cuf in one pass over the interior. */
 int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
                        double *w, const double *w_aux, double *fl, double *fln,
                        const double *coeff_deriv1, const double *dcsidx, const double *detady,
                        const double *dzitdz, int reduced, int chem, int nreactions,
-                       const double *mw, const double *arr_a, const double *arr_b,
-                       const double *arr_ea, const double *tb_eff, const double *falloff_coeffs,
-                       const int *reac_ty, const int *isrev, const double *r_coeffs,
-                       const double *p_coeffs, const double *kc_tab, int num_t_tab,
-                       double t_min_tab, double dt_tab, double r_univ, double gamdt,
-                       const int *fluid_mask, void *stream);
+                       const double *mw, const double *kin_pack, const double *kc_tab,
+                       int num_t_tab, double t_min_tab, double dt_tab, double r_univ,
+                       double gamdt, const int *fluid_mask, void *stream);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ _ARGS = {
     "update_field": "iiiii" "ppp" "s",
     "compute_dt": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
     "rhs_sweep": "iiiiiiiii" "ppppp" "p" "iiii" "d" "pp" "i" "p" "ii" "p" "ddd" "ii" "d" "i" "d" "pp" "s",
-    "rhs_finish": "iiiiiii" "pppppppp" "iii" "ppppppppppp" "i" "dddd" "p" "s",
+    "rhs_finish": "iiiiiii" "pppppppp" "iii" "ppp" "i" "dddd" "p" "s",
 }
 
 SIGNATURES: Dict[str, str] = dict(_ARGS)
@@ -125,6 +125,28 @@ def load(path: os.PathLike = LIBPATH) -> Library:
         if missing:
             raise StreamsError(f"libstreams_hip.so lacks entry points: {missing}")
     return _LIB
+
+
+def kinetics_pack(tables, lib: "Library" = None):
+    """Host call of streams_kinetics_pack: the compact per-reaction records (numpy, host) of
+    a Tables object's kinetics, to be copied to the device once."""
+    import numpy as np
+    lib = lib or load()
+    sym = lib.lib.streams_kinetics_pack
+    sym.restype = ctypes.c_int
+    sym.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 11 + [ctypes.c_int]
+    t = tables
+    f = lambda a, dt=np.float64: np.ascontiguousarray(np.asarray(a, dtype=dt).T)  # noqa: E731
+    arrs = [f(t.arr_a), f(t.arr_b), f(t.arr_ea), f(t.tb_eff), f(t.falloff_coeffs), f(t.reac_ty, np.int32),
+            f(t.isRev, np.int32), f(t.r_coeffs), f(t.p_coeffs), f(t.mw)]
+    need = sym(t.n_species, t.nreactions, *(a.ctypes.data for a in arrs), None, 0)
+    if need < 0:
+        raise StreamsError(f"streams_kinetics_pack returned {need}: {STATUS.get(need, '')}")
+    out = np.zeros(max(need, 1))
+    rc = sym(t.n_species, t.nreactions, *(a.ctypes.data for a in arrs), out.ctypes.data, need)
+    if rc != need:
+        raise StreamsError(f"streams_kinetics_pack returned {rc}: {STATUS.get(rc, '')}")
+    return out
 
 
 def species_supported() -> Tuple[int, ...]:

@@ -4,7 +4,7 @@
 // fastest so that every load and store of a wave touches 64 consecutive doubles of one
 // variable plane (coalesced 512-byte segments).  Pairwise Wilke constants that depend on
 // the molar masses only are built once per workgroup in LDS.
-#include "common.h"
+#include "thermo_dev.h"
 
 namespace streams {
 
@@ -17,14 +17,16 @@ struct AuxTables {
   const int *nainsp;
 };
 
-template <int NS>
-__global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int jstart, int kstart,
+template <int NS, int LO>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) eval_aux_kernel(Box b, int istart, int jstart, int kstart,
                                                        int ni, int nj, long npts, double *__restrict__ w,
                                                        double *__restrict__ w_aux, Thermo th,
                                                        AuxTables tb, double tol_iter_nr) {
   using X = Idx<NS>;
   __shared__ double c_inv[NS * NS];  // 1/sqrt(1 + W_l/W_m)
   __shared__ double c_pow[NS * NS];  // (W_m/W_l)^(1/4)
+  __shared__ double tlds[thermo_lds_doubles<NS>(TH_CP | TH_CV | TH_CVH)];
+  const ThermoLds TL = thermo_stage<NS, LO>(th, tlds, TH_CP | TH_CV | TH_CVH);
   for (int q = threadIdx.x; q < NS * NS; q += blockDim.x) {
     int l = q % NS, m = q / NS;
     c_inv[q] = 1.0 / sqrt(1.0 + tb.mw[l] * tb.mwinv[m]);
@@ -62,11 +64,14 @@ __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int js
   const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
   const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
   const double ee = rhoe * ri - qq;
-  const double tt = temperature_from_e<NS>(ee, w_aux[pt + ps * (X::J_T - 1)], th, tol_iter_nr, yy);
+  int nrange[NS];
+  const double tt = temperature_from_e_lds<NS, LO>(ee, w_aux[pt + ps * (X::J_T - 1)], TL, tol_iter_nr, yy, nrange);
   double rmix = tb.rgas[0] * yy[0];
 #pragma unroll
   for (int s = 1; s < NS; ++s) rmix = rmix + tb.rgas[s] * yy[s];
-  const double cploc = mixture_cp<NS>(tt, th, yy);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) nrange[s] = range_of<NS>(TL, s, tt);
+  const double cploc = mixture_cp_lds<NS, LO>(TL, nrange, yy, TPow<LO>(tt));
   const double gamloc = cploc / (cploc - rmix);
   const double pp = rho * tt * rmix;
   const double c = sqrt(gamloc * pp * ri);
@@ -123,7 +128,7 @@ __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int js
     smu[s] = sqrt(mus[s]);
   }
   double mu = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int l = 0; l < NS; ++l) {
     const double isl = 1.0 / smu[l];
     double den = 0.0;
@@ -140,7 +145,7 @@ __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int js
   w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond;
   // Bird species diffusion into the mixture
   const double fac = tb.p0 / pp * rho;
-#pragma unroll
+#pragma unroll 1
   for (int l = 0; l < NS; ++l) {
     double dden = 0.0;
 #pragma unroll
@@ -220,14 +225,16 @@ extern "C" int streams_eval_aux(int n_s, int nx, int ny, int nz, int ng, int nv,
       kend > nz + ng)
     return STREAMS_E_DIMS;
   if (num_t_tab < 1) return STREAMS_E_DIMS;
+  if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;
   Box b{nx, ny, nz, ng};
   Thermo th{cv_coeff, cp_coeff, trange, nsetcv, indx_cp_l, indx_cp_r};
   AuxTables tb{rgas, mw, mwinv, visc_species, lambda_species, diffbin_species, num_t_tab, t_min_tab,
                dt_tab, p0, r_univ, n_eoi, aw_eoi, beta0, coeff_eoi, nainsp};
   int ni = iend - istart + 1, nj = jend - jstart + 1, nk = kend - kstart + 1;
   long npts = (long)ni * nj * nk;
-  STREAMS_DISPATCH_NS(n_s, (eval_aux_kernel<NS><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
-                                b, istart, jstart, kstart, ni, nj, npts, w, w_aux, th, tb, tol_iter_nr)));
+  STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r,
+                               (eval_aux_kernel<NS, LO><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
+                                   b, istart, jstart, kstart, ni, nj, npts, w, w_aux, th, tb, tol_iter_nr))));
   return hip_status();
 }
 

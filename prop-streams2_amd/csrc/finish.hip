@@ -7,25 +7,28 @@
 // w_aux as written by eval_aux in the same substep: w is unchanged since then, so rho and
 // Y are the identical numbers compute_chemistry_cuf recomputes; its extra Newton iteration
 // moves T by less than the 1e-12 K stopping tolerance.
-#include "chem_common.h"
+#include "kinpack.h"
 
 namespace streams {
 
 struct FinishArgs {
-  double *w;
-  const double *w_aux;
-  double *fl, *fln;
   const double *cd1, *dcsidx, *detady, *dzitdz;
   const int *mask;
-  Kin K;
+  const double *mw;
+  KinPacked K;
   int lmax, reduced, chem;
   double gamdt;
 };
 
+// One interior cell per thread, i fastest, so that every access of a wave is a coalesced
+// 512-byte segment of one variable plane.  The species part (chemistry, then the RK
+// update of the species equations) and the momentum/energy part (reduced viscous terms,
+// then their RK update) run one after the other to keep few values live at a time.
 template <int NS>
-__global__ void __launch_bounds__(256) finish_kernel(Box b, FinishArgs F) {
+__global__ void __launch_bounds__(256) finish_kernel(Box b, double *__restrict__ w, const double *__restrict__ wa,
+                                                     double *__restrict__ fl, double *__restrict__ fln,
+                                                     FinishArgs F) {
   using X = Idx<NS>;
-  constexpr int NV = NS + 4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= b.fstride()) return;
   const int i = 1 + (int)(t % b.nx);
@@ -33,10 +36,33 @@ __global__ void __launch_bounds__(256) finish_kernel(Box b, FinishArgs F) {
   const int j = 1 + (int)(r % b.ny);
   const int k = 1 + (int)(r / b.ny);
   const long pt = b.p(i, j, k), ps = b.pstride(), fs_ = b.fstride();
-  const double *wa = F.w_aux;
-  double acc[NV];
+  const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
+  const double gamdt = F.gamdt;
+  // ---- species: explicit chemistry (compute_chemistry_cuf) and the RK update
+  {
+    double acc[NS];
 #pragma unroll
-  for (int m = 0; m < NV; ++m) acc[m] = F.fl[t + fs_ * m];
+    for (int s = 0; s < NS; ++s) acc[s] = fl[t + fs_ * s];
+    if (F.chem) {
+      const double rho = wa[pt + ps * (X::J_R - 1)];
+      const double tt = wa[pt + ps * (X::J_T - 1)];
+      double conc[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) conc[s] = rho * wa[pt + ps * s] / F.mw[s];
+      chemistry_source_packed<NS>(F.K, tt, conc, acc);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      fl[t + fs_ * s] = acc[s];
+      const double fn = fln[t + fs_ * s] - gamdt * acc[s];
+      fln[t + fs_ * s] = fn;
+      if (fluid) w[pt + ps * s] = w[pt + ps * s] + fn;
+    }
+  }
+  // ---- momentum and energy: reduced viscous terms (visflx_reduced_cuf) and the RK update
+  double acc[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) acc[m] = fl[t + fs_ * (NS + m)];
   if (F.reduced) {
     const long sx = 1, sy = b.px(), sz = b.px() * b.py();
     const double *U = wa + ps * (X::J_U - 1), *V = wa + ps * (X::J_V - 1), *Wv = wa + ps * (X::J_W - 1);
@@ -76,26 +102,18 @@ __global__ void __launch_bounds__(256) finish_kernel(Box b, FinishArgs F) {
     const double sigah = (s11 * ux + s12 * uy + s13 * uz + s21 * vx + s22 * vy + s23 * vz + s31 * wx + s32 * wy +
                           s33 * wz) * mu;
     const double sigq = sigx * uu + sigy * vv + sigz * ww + sigah;
-    acc[X::I_U - 1] = acc[X::I_U - 1] - sigx;
-    acc[X::I_V - 1] = acc[X::I_V - 1] - sigy;
-    acc[X::I_W - 1] = acc[X::I_W - 1] - sigz;
-    acc[X::I_E - 1] = acc[X::I_E - 1] - sigq;
+    acc[0] = acc[0] - sigx;
+    acc[1] = acc[1] - sigy;
+    acc[2] = acc[2] - sigz;
+    acc[3] = acc[3] - sigq;
   }
-  if (F.chem) {
-    const double rho = wa[pt + ps * (X::J_R - 1)];
-    const double tt = wa[pt + ps * (X::J_T - 1)];
-    double conc[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) conc[s] = rho * wa[pt + ps * s] / F.K.mw[s];
-    chemistry_source<NS>(F.K, tt, conc, acc);
-  }
-  const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
-#pragma unroll
-  for (int m = 0; m < NV; ++m) {
-    F.fl[t + fs_ * m] = acc[m];
-    const double fn = F.fln[t + fs_ * m] - F.gamdt * acc[m];
-    F.fln[t + fs_ * m] = fn;
-    if (fluid) F.w[pt + ps * m] = F.w[pt + ps * m] + fn;
+  for (int m = 0; m < 4; ++m) {
+    const long q = t + fs_ * (NS + m);
+    fl[q] = acc[m];
+    const double fn = fln[q] - gamdt * acc[m];
+    fln[q] = fn;
+    if (fluid) w[pt + ps * (NS + m)] = w[pt + ps * (NS + m)] + fn;
   }
 }
 
@@ -103,27 +121,80 @@ __global__ void __launch_bounds__(256) finish_kernel(Box b, FinishArgs F) {
 
 using namespace streams;
 
+// Host packer of the kinetics records read by the fused chemistry (layout in kinpack.h).
+extern "C" int streams_kinetics_pack(int n_s, int nreactions, const double *arr_a, const double *arr_b,
+                                     const double *arr_ea, const double *tb_eff, const double *falloff_coeffs,
+                                     const int *reac_ty, const int *isrev, const double *r_coeffs,
+                                     const double *p_coeffs, const double *mw, double *out, int cap) {
+  if (n_s < 1 || n_s > KTB) return STREAMS_E_NSPECIES;
+  if (nreactions < 0) return STREAMS_E_DIMS;
+  const long need = (long)nreactions * KREC;
+  if (out == nullptr) return (int)need;
+  if (cap < need) return STREAMS_E_DIMS;
+  const int nr = nreactions;
+  for (int r = 0; r < nr; ++r) {
+    double *R = out + (long)KREC * r;
+    for (int q = 0; q < KREC; ++q) R[q] = 0.0;
+    int *I = reinterpret_cast<int *>(R);
+    I[KI_TY] = reac_ty[r];
+    I[KI_REV] = isrev[r];
+    int nre = 0, npr = 0, nnet = 0;
+    for (int s = 0; s < n_s; ++s) {
+      const double rr = r_coeffs[r + (long)nr * s], pp = p_coeffs[r + (long)nr * s];
+      if (rr != 0.0) {
+        if (nre == KSIDE) return STREAMS_E_OPTION;
+        I[KI_RE + nre] = s;
+        R[KD_REX + nre] = rr;
+        ++nre;
+      }
+      if (pp != 0.0) {
+        if (npr == KSIDE) return STREAMS_E_OPTION;
+        I[KI_PR + npr] = s;
+        R[KD_PEX + npr] = pp;
+        ++npr;
+      }
+      if (pp - rr != 0.0) {
+        if (nnet == KNET) return STREAMS_E_OPTION;
+        I[KI_NET + nnet] = s;
+        R[KD_NET + nnet] = (pp - rr) * mw[s];
+        ++nnet;
+      }
+      R[KD_TB + s] = tb_eff[r + (long)nr * s];
+    }
+    I[KI_NRE] = nre;
+    I[KI_NPR] = npr;
+    I[KI_NNET] = nnet;
+    I[KI_ARR] = (arr_b[r] != 0.0 || arr_ea[r] != 0.0) ? 1 : 0;
+    R[KD_A] = arr_a[r];
+    R[KD_A + 1] = arr_b[r];
+    R[KD_A + 2] = arr_ea[r];
+    R[KD_AINF] = arr_a[r + nr];
+    R[KD_AINF + 1] = arr_b[r + nr];
+    R[KD_AINF + 2] = arr_ea[r + nr];
+    for (int q = 0; q < 5; ++q) R[KD_FC + q] = falloff_coeffs[r + (long)nr * q];
+    I[KI_T2] = falloff_coeffs[r + 3L * nr] != -3.14 ? 1 : 0;
+  }
+  return (int)need;
+}
+
 // Tail of one RK substep: reduced viscous terms (reduced=1), explicit chemistry (chem=1),
 // update_flux and update_field, in one pass.
 extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order, double *w,
                                   const double *w_aux, double *fl, double *fln, const double *coeff_deriv1,
                                   const double *dcsidx, const double *detady, const double *dzitdz, int reduced,
-                                  int chem, int nreactions, const double *mw, const double *arr_a,
-                                  const double *arr_b, const double *arr_ea, const double *tb_eff,
-                                  const double *falloff_coeffs, const int *reac_ty, const int *isrev,
-                                  const double *r_coeffs, const double *p_coeffs, const double *kc_tab,
-                                  int num_t_tab, double t_min_tab, double dt_tab, double r_univ, double gamdt,
-                                  const int *fluid_mask, void *stream) {
+                                  int chem, int nreactions, const double *mw, const double *kin_pack,
+                                  const double *kc_tab, int num_t_tab, double t_min_tab, double dt_tab,
+                                  double r_univ, double gamdt, const int *fluid_mask, void *stream) {
   if (nv_aux < 2 * n_s + 13) return STREAMS_E_DIMS;
   const int lmax = visc_order / 2;
   if (reduced && (lmax < 1 || lmax > 4 || lmax > ng)) return STREAMS_E_OPTION;
+  if (n_s > KTB) return STREAMS_E_NSPECIES;
   Box b{nx, ny, nz, ng};
   const long n = b.fstride();
   if (n == 0) return 0;
-  FinishArgs F{w, w_aux, fl, fln, coeff_deriv1, dcsidx, detady, dzitdz, fluid_mask,
-               Kin{mw, arr_a, arr_b, arr_ea, tb_eff, falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev,
-                   nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
+  FinishArgs F{coeff_deriv1, dcsidx, detady, dzitdz, fluid_mask, mw,
+               KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                lmax, reduced, chem && nreactions > 0, gamdt};
-  STREAMS_DISPATCH_NS(n_s, (finish_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, F)));
+  STREAMS_DISPATCH_NS(n_s, (finish_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F)));
   return hip_status();
 }

@@ -1,81 +1,163 @@
-// sweep.hip — fused convective + viscous sweep along one direction, LDS-tiled.
+// sweep.hip — fused convective + viscous sweep along one direction, LDS-tiled, in rounds.
 //
-// One workgroup of 256 threads owns a tile of C columns x S1 faces (S1-1 cells) of one
-// direction.  It stages the convective stencil variables (Y_s, u, v, w, h, p, rho) of the
-// tile and its halo in LDS with coalesced loads, then each thread evaluates one face:
-// the KEEP split-form flux from LDS (or the WENO flux parked in fhat by the shock
-// pre-pass) and the staggered viscous / diffusive flux (the two cells of the face, the
-// transport-only variables read straight from HBM).  Face fluxes meet in LDS and each
-// thread updates its cell:
+// A workgroup of 256 threads owns a tile of C columns x S1 faces (S1-1 cells updated) of
+// one direction, one thread per face.  The KEEP split-form central flux (the central branch
+// of euler_{x,y,z}_hybrid, kernels_gpu.F90:150-337) is evaluated in rounds that each stage
+// part of the stencil variables in LDS with coalesced loads:
+//   round 1: rho, u, v, w, h, p -> the pair factors (rho_a+rho_b)(un_a+un_b)*KEEP(n), kept in
+//            registers, and the momentum / energy fluxes;
+//   round 2..: the mass fractions, KCH species at a time -> the species fluxes.
+// Each round's face fluxes meet in a small LDS exchange buffer and are differenced into the
+// thread's cell right away, so only the cell's accumulator stays live between rounds.  The
+// viscous / diffusive flux of the face (visflx_{x,y,z}_cuf, :3658-4031) then reads its two
+// cells from HBM (cache hits: each cell is read by two faces) with the species enthalpies
+// from an LDS copy of the cp tables, and is exchanged the same way.  Faces flagged by the
+// shock sensor take the WENO flux that the pre-pass parked in fhat.  One read-modify-write
+// of fl per cell:
 //     fl += (Fc(c) - Fc(c-1)) * dxi(c) + 2/(x(c+1)-x(c-1)) * (Fv(c) - Fv(c-1))
 // which is euler_{x,y,z}_update_cuf followed by the update of visflx_{x,y,z}_cuf
-// (kernels_gpu.F90:732-793, :3775-3788) in one read-modify-write of fl.  In the first
-// sweep of a substep the RK bookkeeping of init_flux_cuf (:28-45) is folded in:
-// fln = -rhodt * fl_old and fl starts from this sweep's contribution.
+// (:732-793, :3775-3788).  In the first sweep of a substep the RK bookkeeping of
+// init_flux_cuf (:28-45) is folded in: fln = -rhodt * fl_old and fl = this sweep's part.
 //
-// Tiles: x (lanes along the sweep): C = 8 lines x S1 = 32 faces; y, z (lanes along i):
-// C = 16 columns x S1 = 16 faces.  LDS layout keeps the lane index fastest.
+// Tiles: x (faces along the lanes): C = 8 lines x S1 = 32 faces; y, z (lanes along i, the
+// contiguous index): C = 16 columns x S1 = 16 faces.  LDS layout keeps the lane index
+// fastest.
 #include "euler_dev.h"
+#include "thermo_dev.h"
 #include "visc_dev.h"
 
 namespace streams {
 
 template <int DIR> struct TileCfg { static constexpr int C = 16, S1 = 16; };
 template <> struct TileCfg<1> { static constexpr int C = 8, S1 = 32; };
-constexpr int LMX = 4;  // largest half stencil (ep_order 8)
+constexpr int LMX = 4;                      // largest half stencil (ep_order 8)
+constexpr int NPF = LMX * (LMX + 1) / 2;    // pair factors for lmax <= LMX
+constexpr int KCH = 6;                      // LDS slots per staging round
 
 struct SweepArgs {
   EulerArgs E;
   ViscArgs V;
-  double *fln;
   double rhodt;
   int init;          // fold init_flux: fln = -rhodt*fl_old, fl = contribution
   int visc;          // add the viscous flux
   int check_shock;   // sensor threshold below 1: consult the pre-pass results
 };
 
-// staged slots: Y_1..Y_NS, u, v, w, h, p, rho
-template <int NS>
-__device__ constexpr int slot_of(int m) {
-  return m <= NS + 4 ? m - 1 : (m == NS + 6 ? NS + 4 : NS + 5);
+template <int DIR>
+struct Tile {
+  static constexpr int C = TileCfg<DIR>::C, S1 = TileCfg<DIR>::S1, R = S1 + 2 * LMX - 1;
+  static constexpr int SLOT = C * R;        // doubles per staged variable
+  static constexpr int XSLOT = C * S1;      // doubles per exchanged variable
+  __device__ static inline int at(int c, int r) { return DIR == 1 ? c * R + r : r * C + c; }
+  __device__ static inline int xat(int c, int fl) { return DIR == 1 ? c * S1 + fl : fl * C + c; }
+};
+
+// Pair factors and the momentum / energy fluxes of face f from the round-1 slots
+// (0 rho, 1 u, 2 v, 3 w, 4 h, 5 p); rows are cell positions minus p0.
+template <int DIR>
+__device__ inline void keep_round1(const double *sb, int c, int f, int p0, int lmax, int nkeep, const double *cd1,
+                                   bool zero, double *pf, double *fme) {
+  using T = Tile<DIR>;
+  auto V = [&](int slot, int p) { return sb[slot * T::SLOT + T::at(c, p - p0)]; };
+  constexpr int SN = DIR == 1 ? 1 : DIR == 2 ? 2 : 3;   // slot of the normal velocity
+  double ft2 = 0, ft3 = 0, ft4 = 0, ft5 = 0, ft6 = 0;
+  int q = 0;
+  for (int l = 1; l <= lmax; ++l) {
+    double uvs2 = 0, uvs3 = 0, uvs4 = 0, uvs5 = 0, uvs5_i = 0, uvs5_k = 0, uvs5_p = 0, uvs6 = 0;
+    for (int m = 0; m <= l - 1; ++m, ++q) {
+      const int a = f - m, b = f - m + l;
+      const double rhoi = V(0, a), uui = V(1, a), vvi = V(2, a), wwi = V(3, a), enti = V(4, a), ppi = V(5, a);
+      const double rhoip = V(0, b), uuip = V(1, b), vvip = V(2, b), wwip = V(3, b), entip = V(4, b), ppip = V(5, b);
+      const double uni = SN == 1 ? uui : SN == 2 ? vvi : wwi;
+      const double unip = SN == 1 ? uuip : SN == 2 ? vvip : wwip;
+      const double rhom = rhoi + rhoip;
+      if (nkeep >= 0) {
+        double drhof = 1.0;
+        if (nkeep > 0) {
+          const double drho = 2.0 * (rhoip - rhoi) / rhom;
+          double ts = (0.5 * drho) * (0.5 * drho);
+          const double t2 = ts;
+          double sden = 1.0 + ts / 3.0;
+          for (int n = 2; n <= nkeep; ++n) { ts = t2 * ts; sden = sden + ts / (1.0 + 2 * n); }
+          drhof = 1.0 / sden;
+        }
+        const double uv_part = (uni + unip) * rhom * drhof;
+        pf[q] = uv_part;
+        const double eei = enti - ppi / rhoi - 0.5 * (uui * uui + vvi * vvi + wwi * wwi);
+        const double eeip = entip - ppip / rhoip - 0.5 * (uuip * uuip + vvip * vvip + wwip * wwip);
+        const double eem = eei + eeip;
+        double deef = 1.0;
+        if (nkeep > 0) {
+          const double dee = 2.0 * (eeip - eei) / eem;
+          double te = (0.5 * dee) * (0.5 * dee);
+          const double te2 = te;
+          double sde = 1.0 + te, sne = 1.0 + te / 3.0;
+          for (int n = 2; n <= nkeep; ++n) { te = te2 * te; sde = sde + te; sne = sne + te / (1.0 + 2 * n); }
+          deef = sne / sde;
+        }
+        uvs2 = uvs2 + uv_part * (uui + uuip);
+        uvs3 = uvs3 + uv_part * (vvi + vvip);
+        uvs4 = uvs4 + uv_part * (wwi + wwip);
+        uvs5_i = uvs5_i + uv_part * eem * deef;
+        uvs5_k = uvs5_k + uv_part * (uui * uuip + vvi * vvip + wwi * wwip);
+        uvs5_p = uvs5_p + 4.0 * (uni * ppip + unip * ppi);
+        uvs6 = uvs6 + 2.0 * (ppi + ppip);
+      } else {
+        const double uv_part = (uni + unip) * rhom;
+        pf[q] = uv_part;
+        uvs2 = uvs2 + uv_part * (uui + uuip);
+        uvs3 = uvs3 + uv_part * (vvi + vvip);
+        uvs4 = uvs4 + uv_part * (wwi + wwip);
+        uvs5 = uvs5 + uv_part * (enti + entip);
+        uvs6 = uvs6 + 2.0 * (ppi + ppip);
+      }
+    }
+    const double cl = cd1[(l - 1) + 4 * (lmax - 1)];
+    ft2 = ft2 + cl * uvs2;
+    ft3 = ft3 + cl * uvs3;
+    ft4 = ft4 + cl * uvs4;
+    ft5 = ft5 + cl * (nkeep >= 0 ? (uvs5_i + uvs5_k + uvs5_p) : uvs5);
+    ft6 = ft6 + cl * uvs6;
+  }
+  fme[0] = zero ? 0.0 : 0.25 * ft2;
+  fme[1] = zero ? 0.0 : 0.25 * ft3;
+  fme[2] = zero ? 0.0 : 0.25 * ft4;
+  fme[3] = zero ? 0.0 : 0.25 * ft5;
+  fme[SN - 1] = fme[SN - 1] + 0.5 * ft6;
 }
 
-template <int NS, int DIR>
-struct LdsLine {
-  static constexpr int C = TileCfg<DIR>::C, S1 = TileCfg<DIR>::S1, R = S1 + 2 * LMX - 1;
-  const double *lds;
-  int c, p0;
-  __device__ inline int at(int slot, int r) const {
-    return DIR == 1 ? slot * C * R + c * R + r : slot * C * R + r * C + c;
+// species flux of face f from one staged mass-fraction slot
+template <int DIR>
+__device__ inline double keep_species(const double *slot, int c, int f, int p0, int lmax, const double *cd1,
+                                      const double *pf, bool zero) {
+  using T = Tile<DIR>;
+  double ft1 = 0.0;
+  int q = 0;
+  for (int l = 1; l <= lmax; ++l) {
+    double uvs1 = 0.0;
+    for (int m = 0; m <= l - 1; ++m, ++q)
+      uvs1 = uvs1 + pf[q] * (slot[T::at(c, f - m - p0)] + slot[T::at(c, f - m + l - p0)]);
+    ft1 = ft1 + cd1[(l - 1) + 4 * (lmax - 1)] * uvs1;
   }
-  __device__ inline double operator()(int p, int m) const { return lds[at(slot_of<NS>(m), p - p0)]; }
-};
+  return zero ? 0.0 : 0.25 * ft1;
+}
 
-// viscous stencil access: staged variables from LDS, transport-only ones from HBM
-template <int NS, int DIR>
-struct MixLine {
-  LdsLine<NS, DIR> s;
-  Line g;
-  __device__ inline double operator()(int p, int m) const {
-    using X = Idx<NS>;
-    if (m <= NS + 3) return s(p, m);   // Y, u, v, w
-    return g(p, m);                    // T, mu, k, rho*D
-  }
-};
-
-template <int NS, int DIR>
-__global__ void __launch_bounds__(256) sweep_kernel(Box b, SweepArgs A) {
+template <int NS, int DIR, int LO>
+__global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restrict__ wa, double *__restrict__ fl,
+                                                    double *__restrict__ fln, const double *__restrict__ fhat,
+                                                    SweepArgs A) {
   using X = Idx<NS>;
-  constexpr int NV = NS + 4, NSLOT = NS + 6;
-  constexpr int C = TileCfg<DIR>::C, S1 = TileCfg<DIR>::S1, R = S1 + 2 * LMX - 1;
-  constexpr int STAGE = NSLOT * C * R;
-  constexpr int EXCH = NV * C * S1;
-  __shared__ double lds[STAGE > EXCH ? STAGE : EXCH];
+  using T = Tile<DIR>;
+  constexpr int NV = NS + 4, C = T::C, S1 = T::S1, R = T::R;
+  __shared__ double sb[KCH * T::SLOT];
+  __shared__ double xb[KCH * T::XSLOT];
+  __shared__ double tlds[thermo_lds_doubles<NS>(TH_CPH)];
   const EulerArgs &E = A.E;
   const int t = threadIdx.x;
   int c, fl_;
   if (DIR == 1) { fl_ = t % S1; c = t / S1; }
   else { c = t % C; fl_ = t / C; }
+  const ThermoLds TL = thermo_stage<NS, LO>(A.V.th, tlds, TH_CPH);
   // tile coordinates
   const int us = E.fs + 1, ue = E.fe;
   const int seg = DIR == 1 ? blockIdx.x : blockIdx.y;
@@ -85,12 +167,13 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, SweepArgs A) {
   const int ncol = DIR == 1 ? b.ny : b.nx;
   const int lmax_b = E.lmax_base;
   const int p0 = s0 - LMX;
-  // ---- stage the convective variables of the tile
-  {
-    const int plo = s0 - lmax_b;
-    int phi = s0 + S1 - 2;
-    phi = (phi > ue ? ue : phi) + lmax_b;
-    const long ps = b.pstride();
+  const long ps = b.pstride();
+  // rows of the tile that any face reads: [plo, phi]
+  const int plo = s0 - lmax_b;
+  int phi = s0 + S1 - 2;
+  phi = (phi > ue ? ue : phi) + lmax_b;
+  // staging of up to KCH variables (reference indices m[k]) into sb
+  auto stage = [&](const int *vars, int nvar) {
     for (int idx = t; idx < C * R; idx += 256) {
       int cc, r;
       if (DIR == 1) { cc = idx / R; r = idx % R; }
@@ -101,73 +184,156 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, SweepArgs A) {
       long base, st;
       line_base<DIR>(b, col, t2, base, st);
       const long g = base + (long)p * st;
-      const int li = DIR == 1 ? cc * R + r : r * C + cc;
-#pragma unroll
-      for (int sl = 0; sl < NSLOT; ++sl) {
-        const int m = sl < NS + 4 ? sl + 1 : (sl == NS + 4 ? X::J_P : X::J_R);
-        lds[sl * C * R + li] = E.w_aux[g + ps * (m - 1)];
-      }
+      const int li = T::at(cc, r);
+      for (int k = 0; k < nvar; ++k) sb[k * T::SLOT + li] = wa[g + ps * (vars[k] - 1)];
     }
-  }
-  __syncthreads();
-  // ---- one face per thread
+  };
   const int col = cbase + c;
   const int f = s0 - 1 + fl_;
   const bool valid = col <= ncol && f <= ue;
-  double fc[NV], fv[NV];
-  if (valid) {
-    Line G{E.w_aux, 0, 0, b.pstride()};
-    line_base<DIR>(b, col, t2, G.base, G.st);
-    LdsLine<NS, DIR> L{lds, c, p0};
-    if (E.check_shock_flag && shocked<NS>(G, f, E.weno_scheme, E.sensor_threshold)) {
-#pragma unroll
-      for (int m = 0; m < NV; ++m) fc[m] = E.fhat[G.base + (long)f * G.st + b.pstride() * m];
-    } else {
-      int lmax = lmax_b + epo_at<DIR>(b, E.epo, col, t2, f);
-      lmax = lmax < 1 ? 1 : lmax;
-      const bool zero = (f == 0 && E.zmin == 1) || (f == E.n && E.zmax == 1);
-      keep_flux<NS, DIR>(L, f, lmax, E.nkeep, E.cd1, zero, fc);
-    }
-    if (A.visc) {
-      MixLine<NS, DIR> M{L, G};
-      visc_face<NS>(M, f, A.V, b.ng, fv);
-    } else {
-#pragma unroll
-      for (int m = 0; m < NV; ++m) fv[m] = 0.0;
-    }
-  }
-  __syncthreads();
-  // ---- convective flux exchange
-  const int me = DIR == 1 ? c * S1 + fl_ : fl_ * C + c;
-  const int left = DIR == 1 ? me - 1 : me - C;
-  if (valid) {
-#pragma unroll
-    for (int m = 0; m < NV; ++m) lds[m * C * S1 + me] = fc[m];
-  }
-  __syncthreads();
   const bool upd = valid && fl_ >= 1;
-  double acc[NV];
-  const int cell = f;
-  if (upd) {
-    const double met = E.metric[cell - 1];
-#pragma unroll
-    for (int m = 0; m < NV; ++m) acc[m] = (fc[m] - lds[m * C * S1 + left]) * met;
+  const int me = T::xat(c, fl_);
+  const int left = DIR == 1 ? me - 1 : me - C;
+  Line G{wa, 0, 0, ps};
+  line_base<DIR>(b, col <= ncol ? col : ncol, t2, G.base, G.st);
+  const bool shock = valid && E.check_shock_flag && shocked<NS>(G, f, E.weno_scheme, E.sensor_threshold);
+  const long fidx = G.base + (long)f * G.st;     // face f in fhat
+  int lmax = 1;
+  bool zero = false;
+  if (valid) {
+    lmax = lmax_b + epo_at<DIR>(b, E.epo, col, t2, f);
+    lmax = lmax < 1 ? 1 : lmax;
+    zero = (f == 0 && E.zmin == 1) || (f == E.n && E.zmax == 1);
   }
-  if (A.visc) {
+  const double met = upd ? E.metric[f - 1] : 0.0;
+  double acc[NV];
+  double pf[NPF];
+  // ---- round 1: rho, u, v, w, h, p
+  {
+    const int vars[6] = {X::J_R, X::J_U, X::J_V, X::J_W, X::J_H, X::J_P};
+    stage(vars, 6);
     __syncthreads();
+    double fme[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
+      if (shock) {
 #pragma unroll
-      for (int m = 0; m < NV; ++m) lds[m * C * S1 + me] = fv[m];
+        for (int k = 0; k < 4; ++k) fme[k] = fhat[fidx + ps * (NS + k)];
+      } else {
+        keep_round1<DIR>(sb, c, f, p0, lmax, E.nkeep, E.cd1, zero, pf, fme);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) xb[k * T::XSLOT + me] = fme[k];
     }
     __syncthreads();
     if (upd) {
-      const double *x = A.V.coord;
-      const double dxhl = 2.0 / (x[cell + 1 + b.ng - 1] - x[cell - 1 + b.ng - 1]);
 #pragma unroll
-      for (int m = 0; m < NV; ++m) acc[m] = acc[m] + dxhl * (fv[m] - lds[m * C * S1 + left]);
+      for (int k = 0; k < 4; ++k) acc[NS + k] = (fme[k] - xb[k * T::XSLOT + left]) * met;
+    }
+  }
+  // ---- species rounds
+#pragma unroll
+  for (int s0k = 0; s0k < NS; s0k += KCH) {
+    const int nk = NS - s0k < KCH ? NS - s0k : KCH;
+    int vars[KCH];
+#pragma unroll
+    for (int k = 0; k < KCH; ++k) vars[k] = s0k + k + 1;
+    __syncthreads();                        // sb and xb free
+    stage(vars, nk);
+    __syncthreads();
+    if (valid) {
+#pragma unroll 1
+      for (int k = 0; k < nk; ++k)
+        xb[k * T::XSLOT + me] = shock ? fhat[fidx + ps * (s0k + k)]
+                                      : keep_species<DIR>(sb + k * T::SLOT, c, f, p0, lmax, E.cd1, pf, zero);
+    }
+    __syncthreads();
+    if (upd) {
+#pragma unroll
+      for (int k = 0; k < KCH; ++k)
+        if (k < nk) acc[s0k + k] = (xb[k * T::XSLOT + me] - xb[k * T::XSLOT + left]) * met;
+    }
+  }
+  // ---- viscous / diffusive flux (visflx_{x,y,z}_cuf), two cells a = f, c = f + 1
+  if (A.visc) {
+    const ViscArgs &V = A.V;
+    const int a = f, cc = f + 1;
+    double sigx = 0, sigy = 0, sigz = 0, sigq = 0, mwm = 1, mwmp = 1, mwmif = 0, fsum = 0, ta = 0, tc = 0;
+    if (valid) {
+      const double uu = G(a, X::J_U), uup = G(cc, X::J_U);
+      const double vv = G(a, X::J_V), vvp = G(cc, X::J_V);
+      const double ww = G(a, X::J_W), wwp = G(cc, X::J_W);
+      ta = G(a, X::J_T);
+      tc = G(cc, X::J_T);
+      const double mu = G(a, X::J_MU), mup = G(cc, X::J_MU);
+      const double kc = G(a, X::J_K_COND), kcp = G(cc, X::J_K_COND);
+      const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+      const double qqp = 0.5 * (uup * uup + vvp * vvp + wwp * wwp);
+      const double dx = V.coord[f + 1 + b.ng - 1] - V.coord[f + b.ng - 1];
+      double muf = mu + mup;
+      muf = 0.5 * muf / dx;
+      double kcf = kc + kcp;
+      kcf = 0.5 * kcf / dx;
+      sigx = (uup - uu) * muf;
+      sigy = (vvp - vv) * muf;
+      sigz = (wwp - ww) * muf;
+      sigq = (tc - ta) * kcf + (qqp - qq) * muf;
+      double rm = V.rgas[0] * G(a, 1), rmp = V.rgas[0] * G(cc, 1);
+#pragma unroll
+      for (int s = 1; s < NS; ++s) {
+        rm = rm + V.rgas[s] * G(a, s + 1);
+        rmp = rmp + V.rgas[s] * G(cc, s + 1);
+      }
+      mwm = V.r_univ / rm;
+      mwmp = V.r_univ / rmp;
+      mwmif = 0.5 * (mwmp + mwm) / (mwmp * mwm);
+      mwmif = mwmif / dx;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const double rd = 0.5 * (G(a, X::J_D_START + s + 1) + G(cc, X::J_D_START + s + 1));
+        fsum = fsum + (G(cc, s + 1) * mwmp - G(a, s + 1) * mwm) * rd;
+      }
+      fsum = fsum * mwmif;
+    }
+    const double dxhl = upd ? 2.0 / (V.coord[f + 1 + b.ng - 1] - V.coord[f - 1 + b.ng - 1]) : 0.0;
+    double et = 0.0;
+#pragma unroll
+    for (int s0k = 0; s0k < NS; s0k += KCH) {
+      const int nk = NS - s0k < KCH ? NS - s0k : KCH;
+      __syncthreads();                      // xb free
+      if (valid) {
+#pragma unroll 1
+        for (int k = 0; k < nk; ++k) {
+          const int s = s0k + k;
+          const double ya = G(a, s + 1), yc = G(cc, s + 1);
+          const double rd = 0.5 * (G(a, X::J_D_START + s + 1) + G(cc, X::J_D_START + s + 1));
+          const double sig = -rd * (yc * mwmp - ya * mwm) * mwmif + fsum * (0.5 * (yc + ya));
+          const double hh = species_h<NS, LO>(TL.cph, s, range_of<NS>(TL, s, ta), TPow<LO>(ta));
+          const double hhp = species_h<NS, LO>(TL.cph, s, range_of<NS>(TL, s, tc), TPow<LO>(tc));
+          et = et + sig * (0.5 * (hh + hhp));
+          xb[k * T::XSLOT + me] = sig;
+        }
+      }
+      __syncthreads();
+      if (upd) {
+#pragma unroll
+        for (int k = 0; k < KCH; ++k)
+          if (k < nk) acc[s0k + k] = acc[s0k + k] + dxhl * (xb[k * T::XSLOT + me] - xb[k * T::XSLOT + left]);
+      }
+    }
+    double fme[4] = {-sigx, -sigy, -sigz, -sigq + et};
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) xb[k * T::XSLOT + me] = fme[k];
+    }
+    __syncthreads();
+    if (upd) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[NS + k] = acc[NS + k] + dxhl * (fme[k] - xb[k * T::XSLOT + left]);
     }
   }
   if (!upd) return;
+  const int cell = f;
   long fcell;
   if (DIR == 1) fcell = b.f(cell, col, t2);
   else if (DIR == 2) fcell = b.f(col, cell, t2);
@@ -176,17 +342,17 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, SweepArgs A) {
   if (A.init) {
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
-      const double old = E.fl[fcell + fs_ * m];
-      A.fln[fcell + fs_ * m] = -A.rhodt * old;
-      E.fl[fcell + fs_ * m] = acc[m];
+      const double old = fl[fcell + fs_ * m];
+      fln[fcell + fs_ * m] = -A.rhodt * old;
+      fl[fcell + fs_ * m] = acc[m];
     }
   } else {
 #pragma unroll
-    for (int m = 0; m < NV; ++m) E.fl[fcell + fs_ * m] = E.fl[fcell + fs_ * m] + acc[m];
+    for (int m = 0; m < NV; ++m) fl[fcell + fs_ * m] = fl[fcell + fs_ * m] + acc[m];
   }
 }
 
-template <int NS, int DIR>
+template <int NS, int DIR, int LO>
 int sweep_launch(const Box &b, const SweepArgs &A, hipStream_t st) {
   constexpr int C = TileCfg<DIR>::C, S1 = TileCfg<DIR>::S1;
   const int ncell = A.E.fe - A.E.fs;
@@ -196,7 +362,8 @@ int sweep_launch(const Box &b, const SweepArgs &A, hipStream_t st) {
   const int ncb = (ncol + C - 1) / C;
   const int nt2 = DIR == 3 ? b.ny : b.nz;
   dim3 grid = DIR == 1 ? dim3(nseg, ncb, nt2) : dim3(ncb, nseg, nt2);
-  sweep_kernel<NS, DIR><<<grid, 256, 0, st>>>(b, A);
+  sweep_kernel<NS, DIR, LO><<<grid, 256, 0, st>>>(b, A.E.w_aux, A.E.fl, A.V.fl == nullptr ? nullptr : A.V.fl,
+                                                  A.E.fhat, A);
   return hip_status();
 }
 
@@ -218,6 +385,7 @@ extern "C" int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, i
   if (dir < 1 || dir > 3) return STREAMS_E_DIMS;
   if (nv_aux < 2 * n_s + 13 || ng < lmax_base || ng < weno_scheme) return STREAMS_E_DIMS;
   if (lmax_base < 1 || lmax_base > LMX || weno_scheme < 1 || weno_scheme > 4) return STREAMS_E_OPTION;
+  if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;
   const int n = dir == 1 ? nx : dir == 2 ? ny : nz;
   if (n < 1) return 0;
   Box b{nx, ny, nz, ng};
@@ -227,8 +395,7 @@ extern "C" int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, i
                   force_zero_flux_min, force_zero_flux_max, weno_scheme, weno_version, flux_splitting,
                   sensor_threshold, tol_iter_nr, rho0, u0, 0, n, n};
   A.E.check_shock_flag = sensor_threshold < 1.0;
-  A.V = ViscArgs{w_aux, coord, rgas, fl, th, r_univ, n};
-  A.fln = fln;
+  A.V = ViscArgs{w_aux, coord, rgas, fln, th, r_univ, n};
   A.rhodt = rhodt;
   A.init = init;
   A.visc = visc;
@@ -238,8 +405,12 @@ extern "C" int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, i
     int rc = weno_prepass_launch(dir, n_s, b, A.E, st);
     if (rc) return rc;
   }
-  if (dir == 1) { STREAMS_DISPATCH_NS(n_s, return (sweep_launch<NS, 1>(b, A, st))); }
-  if (dir == 2) { STREAMS_DISPATCH_NS(n_s, return (sweep_launch<NS, 2>(b, A, st))); }
-  STREAMS_DISPATCH_NS(n_s, return (sweep_launch<NS, 3>(b, A, st)));
+  if (dir == 1) {
+    STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r, return (sweep_launch<NS, 1, LO>(b, A, st))));
+  }
+  if (dir == 2) {
+    STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r, return (sweep_launch<NS, 2, LO>(b, A, st))));
+  }
+  STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r, return (sweep_launch<NS, 3, LO>(b, A, st))));
   return 0;
 }

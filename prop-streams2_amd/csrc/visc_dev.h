@@ -29,8 +29,31 @@ __device__ inline void vline_base(const Box &b, int t1, int t2, long &base, long
   else { base = b.p(t1, t2, 0); st = b.px() * b.py(); }
 }
 
+// Species enthalpies of the two cells of a face from the reference's table layout
+// (get_species_h_from_temperature_dev per species and cell).
+template <int NS>
+struct EnthalpyGlobal {
+  const Thermo &th;
+  double ta, tc;
+  __device__ inline double operator()(int side, int s) const {
+    return species_hcoef<NS>(side == 0 ? ta : tc, th.cp_coeff, th, s + 1);
+  }
+};
+
+template <int NS, class LineT, class HFun>
+__device__ void visc_face_h(const LineT &L, int f, const ViscArgs &A, int ng, double *fh, const HFun &H);
+
 template <int NS, class LineT>
 __device__ void visc_face(const LineT &L, int f, const ViscArgs &A, int ng, double *fh) {
+  using X = Idx<NS>;
+  const EnthalpyGlobal<NS> H{A.th, L(f, X::J_T), L(f + 1, X::J_T)};
+  visc_face_h<NS>(L, f, A, ng, fh, H);
+}
+
+// Staggered viscous / diffusive flux of face f (between cells f and f+1); H(side, s) gives
+// the enthalpy of species s in the left (side 0) or right (side 1) cell.
+template <int NS, class LineT, class HFun>
+__device__ void visc_face_h(const LineT &L, int f, const ViscArgs &A, int ng, double *fh, const HFun &H) {
   using X = Idx<NS>;
   const int a = f, c = f + 1;
   const double uu = L(a, X::J_U), uup = L(c, X::J_U);
@@ -79,8 +102,8 @@ __device__ void visc_face(const LineT &L, int f, const ViscArgs &A, int ng, doub
     const double yyf = 0.5 * (yc[s] + ya[s]);
     const double sig = -rd[s] * (yc[s] * mwmp - ya[s] * mwm) * mwmif + fsum * yyf;
     fh[s] = sig;
-    const double hh = species_hcoef<NS>(tt, A.th.cp_coeff, A.th, s + 1);
-    const double hhp = species_hcoef<NS>(ttp, A.th.cp_coeff, A.th, s + 1);
+    const double hh = H(0, s);
+    const double hhp = H(1, s);
     et = et + sig * (0.5 * (hh + hhp));
   }
   fh[X::I_U - 1] = -sigx;

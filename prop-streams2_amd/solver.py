@@ -326,8 +326,9 @@ class Solver:
                     be.stream)
             first = False
         chem = int(n.enable_chemistry == 1 and self.nr > 0)
-        kin = (self.t_mw, self.t_arr_a, self.t_arr_b, self.t_arr_ea, self.t_tb, self.t_fall, self.t_rty, self.t_rev,
-               self.t_rco, self.t_pco, self.t_kc) if chem else (self.t_mw,) + (None,) * 10
+        if chem and getattr(self, "t_kinpack", None) is None:
+            self.t_kinpack = be.asarray(capi.kinetics_pack(tb, be.lib))
+        kin = (self.t_mw, self.t_kinpack, self.t_kc) if chem else (self.t_mw, None, None)
         be.call("rhs_finish", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, n.visc_order, self.w,
                 self.w_aux, self.fl, self.fln, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], 1, chem, self.nr,
                 *kin, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV, gamdt, None, be.stream)
