@@ -231,11 +231,10 @@ int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, int nz, int 
 /* Kinetics records for the fused chemistry: packs the reference's kinetics tables (HOST
  * arrays in their Fortran layouts, as read by multideal.F90:2763-2926: arr_a/b/ea(nr,2),
  * tb_eff/r_coeffs/p_coeffs(nr,N_S), falloff_coeffs(nr,5), reac_ty/isRev(nr)) and the molar
- * masses into nreactions records of 64 doubles each, the layout documented in
+ * masses into nreactions records of 80 doubles each, the layout documented in
  * csrc/kinpack.h.  Returns the number of doubles of the pack (pass out = NULL to query),
  * STREAMS_E_DIMS if cap is too small, STREAMS_E_OPTION if a reaction has more than 4
- * reactant or product species or 12 species with a net change, STREAMS_E_NSPECIES for
- * N_S > 16.  The caller copies the pack to the device once, as the reference uploads its
+ * reactant or product species, STREAMS_E_NSPECIES for N_S > 16.  The caller copies the pack to the device once, as the reference uploads its
  * tables once (multideal_gpu.F90:1363-1424). */
 int streams_kinetics_pack(int n_s, int nreactions, const double *arr_a, const double *arr_b,
                           const double *arr_ea, const double *tb_eff, const double *falloff_coeffs,

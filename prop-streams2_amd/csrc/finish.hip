@@ -138,7 +138,7 @@ extern "C" int streams_kinetics_pack(int n_s, int nreactions, const double *arr_
     int *I = reinterpret_cast<int *>(R);
     I[KI_TY] = reac_ty[r];
     I[KI_REV] = isrev[r];
-    int nre = 0, npr = 0, nnet = 0;
+    int nre = 0, npr = 0;
     for (int s = 0; s < n_s; ++s) {
       const double rr = r_coeffs[r + (long)nr * s], pp = p_coeffs[r + (long)nr * s];
       if (rr != 0.0) {
@@ -153,17 +153,11 @@ extern "C" int streams_kinetics_pack(int n_s, int nreactions, const double *arr_
         R[KD_PEX + npr] = pp;
         ++npr;
       }
-      if (pp - rr != 0.0) {
-        if (nnet == KNET) return STREAMS_E_OPTION;
-        I[KI_NET + nnet] = s;
-        R[KD_NET + nnet] = (pp - rr) * mw[s];
-        ++nnet;
-      }
+      R[KD_NET + s] = (pp - rr) * mw[s];
       R[KD_TB + s] = tb_eff[r + (long)nr * s];
     }
     I[KI_NRE] = nre;
     I[KI_NPR] = npr;
-    I[KI_NNET] = nnet;
     I[KI_ARR] = (arr_b[r] != 0.0 || arr_ea[r] != 0.0) ? 1 : 0;
     R[KD_A] = arr_a[r];
     R[KD_A + 1] = arr_b[r];

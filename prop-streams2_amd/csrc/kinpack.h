@@ -7,29 +7,29 @@
 // per reaction through the constant address space, which the compiler turns into scalar
 // (SMEM) loads into SGPRs, and touches only the species that take part in the reaction.
 //
-// One record is KREC doubles (512 bytes):
+// One record is KREC doubles (640 bytes):
 //   the first 16 doubles hold 32 int32:
-//     [0] reac_ty   [1] isRev   [2] n_reac   [3] n_prod   [4] n_net
+//     [0] reac_ty   [1] isRev   [2] n_reac   [3] n_prod
 //     [5] 1 when b or Ea of the (low-pressure) rate is nonzero
 //     [6] 1 when the Troe T2 term is present (falloff(4) != -3.14)
-//     [8..11]  reactant species (0-based), [12..15] product species,
-//     [16..27] species with a nonzero net coefficient (0-based, ascending)
+//     [8..11]  reactant species (0-based), [12..15] product species
 //   doubles:
 //     [16..18] A, b, Ea of the rate (the low-pressure limit of a fall-off reaction)
 //     [19..21] A, b, Ea of the high-pressure limit
 //     [22..26] falloff_coeffs(1..5)
 //     [28..31] reactant exponents, [32..35] product exponents
-//     [36..47] (p_s - r_s) * W_s of the n_net species
-//     [48..63] third-body efficiencies of species 1..N_S
+//     [36..51] (p_s - r_s) * W_s of species 1..N_S (zero for species the reaction leaves
+//              unchanged: subtracting q*0 leaves the accumulator as it is)
+//     [52..67] third-body efficiencies of species 1..N_S
 #pragma once
 #include "common.h"
 
 namespace streams {
 
-constexpr int KREC = 64, KSIDE = 4, KNET = 12, KTB = 16;
-constexpr int KI_TY = 0, KI_REV = 1, KI_NRE = 2, KI_NPR = 3, KI_NNET = 4, KI_ARR = 5, KI_T2 = 6;
-constexpr int KI_RE = 8, KI_PR = 12, KI_NET = 16;
-constexpr int KD_A = 16, KD_AINF = 19, KD_FC = 22, KD_REX = 28, KD_PEX = 32, KD_NET = 36, KD_TB = 48;
+constexpr int KREC = 80, KSIDE = 4, KTB = 16;
+constexpr int KI_TY = 0, KI_REV = 1, KI_NRE = 2, KI_NPR = 3, KI_ARR = 5, KI_T2 = 6;
+constexpr int KI_RE = 8, KI_PR = 12;
+constexpr int KD_A = 16, KD_AINF = 19, KD_FC = 22, KD_REX = 28, KD_PEX = 32, KD_NET = 36, KD_TB = 52;
 
 typedef __attribute__((address_space(4))) const double kdouble;
 typedef __attribute__((address_space(4))) const int kint;
@@ -116,7 +116,7 @@ __device__ inline void chemistry_source_packed(const KinPacked &K, double tt, co
       kb = kf / (kp[0] + (kp[1] - kp[0]) * dtt);
     }
     double q1 = 1.0, q2 = 1.0;
-    const int nre = I[KI_NRE], npr = I[KI_NPR], nnet = I[KI_NNET];
+    const int nre = I[KI_NRE], npr = I[KI_NPR];
 #pragma unroll
     for (int e = 0; e < KSIDE; ++e)
       if (e < nre) q1 = q1 * conc_power(conc[I[KI_RE + e]], R[KD_REX + e]);
@@ -125,11 +125,7 @@ __device__ inline void chemistry_source_packed(const KinPacked &K, double tt, co
       if (e < npr) q2 = q2 * conc_power(conc[I[KI_PR + e]], R[KD_PEX + e]);
     const double q = tbv * (kf * q1 - kb * q2);
 #pragma unroll
-    for (int e = 0; e < KNET; ++e)
-      if (e < nnet) {
-        const int s = I[KI_NET + e];
-        acc[s] = acc[s] - q * R[KD_NET + e];
-      }
+    for (int s = 0; s < NS; ++s) acc[s] = acc[s] - q * R[KD_NET + s];
   }
 }
 

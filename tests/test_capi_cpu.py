@@ -53,7 +53,7 @@ def test_kinetics_pack_restates_the_dense_tables(pkg):
                  pkg.mechanism.Mechanism(str(ROOT / "tests/golden/mechanisms/keromnes2013_14sp.yaml"))):
         tb = mech.tables(transport=False)
         ns, nr = tb.n_species, tb.nreactions
-        pack = pkg.capi.kinetics_pack(tb).reshape(nr, 64)
+        pack = pkg.capi.kinetics_pack(tb).reshape(nr, 80)
         ints = pack[:, :16].copy().view(np.int32)
         r_co, p_co = np.zeros((nr, ns)), np.zeros((nr, ns))
         for r in range(nr):
@@ -62,14 +62,11 @@ def test_kinetics_pack_restates_the_dense_tables(pkg):
                 r_co[r, ints[r, 8 + e]] = pack[r, 28 + e]
             for e in range(ints[r, 3]):
                 p_co[r, ints[r, 12 + e]] = pack[r, 32 + e]
-            net = np.zeros(ns)
-            for e in range(ints[r, 4]):
-                net[ints[r, 16 + e]] = pack[r, 36 + e]
-            np.testing.assert_array_equal(net, (tb.p_coeffs[r] - tb.r_coeffs[r]) * tb.mw)
+            np.testing.assert_array_equal(pack[r, 36:36 + ns], (tb.p_coeffs[r] - tb.r_coeffs[r]) * tb.mw)
             np.testing.assert_array_equal(pack[r, 16:19], [tb.arr_a[r, 0], tb.arr_b[r, 0], tb.arr_ea[r, 0]])
             np.testing.assert_array_equal(pack[r, 19:22], [tb.arr_a[r, 1], tb.arr_b[r, 1], tb.arr_ea[r, 1]])
             np.testing.assert_array_equal(pack[r, 22:27], tb.falloff_coeffs[r])
-            np.testing.assert_array_equal(pack[r, 48:48 + ns], tb.tb_eff[r])
+            np.testing.assert_array_equal(pack[r, 52:52 + ns], tb.tb_eff[r])
             assert ints[r, 5] == int(tb.arr_b[r, 0] != 0 or tb.arr_ea[r, 0] != 0)
             assert ints[r, 6] == int(tb.falloff_coeffs[r, 3] != -3.14)
         np.testing.assert_array_equal(r_co, tb.r_coeffs)
