@@ -1,8 +1,8 @@
 ! dropin_check.F90 -- calls the gfx950 library through the Fortran drop-in module exactly as
 ! the reference's host code calls its kernel module: device arrays allocated with hipMalloc
 ! and passed as c_ptr handles, scalars by value.  Checks the RK bookkeeping
-! (init_flux_cuf, update_flux_cuf, update_field_cuf) and the periodic halo exchange (bcswap)
-! against the same arithmetic on the host, and prints "dropin_check: ok".
+! (init_flux_cuf, update_flux_cuf, update_field_cuf) and the periodic halo exchange of the
+! face layers (bcswap) against the same arithmetic on the host, and prints "dropin_check: ok".
 #ifndef N_S
 #define N_S 9
 #endif
@@ -40,8 +40,8 @@ program dropin_check
   integer(c_int), target :: periodic(3), list(64)
   type(c_ptr) :: dw, dfl, dfln
   integer(c_size_t) :: bw, bf
-  integer :: i, j, k, m, nsp, ii, jj, kk
-  real(c_double) :: err
+  integer :: i, j, k, m, nsp, ii, jj, kk, nout
+  real(c_double) :: err, expect
 
   nsp = streams_species_supported(c_loc(list), 64_c_int)
   if (.not. any(list(1:nsp) == N_S)) then
@@ -93,7 +93,15 @@ program dropin_check
         jj = modulo(j - 1, ny) + 1
         do i = 1 - ng, nx + ng
           ii = modulo(i - 1, nx) + 1
-          if (abs(w_d(i, j, k, m) - w_ref(ii, jj, kk, m)) > 1d-15) then
+          ! bcswap fills the six face layers (base_gpu.F90:155-207); edge and corner ghosts
+          ! keep their values
+          nout = merge(1, 0, i < 1 .or. i > nx) + merge(1, 0, j < 1 .or. j > ny) + merge(1, 0, k < 1 .or. k > nz)
+          if (nout <= 1) then
+            expect = w_ref(ii, jj, kk, m)
+          else
+            expect = w(i, j, k, m)
+          end if
+          if (abs(w_d(i, j, k, m) - expect) > 1d-15) then
             write(*, '(a,4i4)') 'dropin_check: field/halo mismatch at ', i, j, k, m
             error stop 1
           end if
