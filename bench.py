@@ -117,6 +117,36 @@ def copy_probe(gib: float):
     print(json.dumps({"copy_bytes_each_way": 8 * n, "ms": ms, "GBps_read_plus_write": 16 * n / ms / 1e6}))
 
 
+def reactor_bench(pkg, n: int, reps: int = 3):
+    """Batched constant-volume reactors (T log-uniform 800-2500 K, p 0.5-20 atm, Dirichlet
+    Y, horizon 1e-6 s): reactor-solves/s of the plain FP64 Rosenbrock batch."""
+    import torch
+    tables = pkg.cases.h2_air_mechanism().tables()
+    w, tg, dttry, tend, info = pkg.reactors.reactor_inputs(tables, n)
+    be = pkg.solver.HipBackend("cuda:0")
+    G = pkg.reactors.ReactorBatchGPU(be, tables, tol=1e-8)
+    d_dt, d_end = be.asarray(dttry), be.asarray(tend)
+    na, nr = be.zeros((n,), "i4"), be.zeros((n,), "i4")
+    times = []
+    for rep in range(reps + 1):
+        wd, td = be.asarray(w), be.asarray(tg)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        G.integrate(wd, td, d_dt, d_end, na, nr)
+        e1.record()
+        torch.cuda.synchronize()
+        if rep > 0:
+            times.append(e0.elapsed_time(e1) / 1e3)
+    times.sort()
+    t = times[len(times) // 2]
+    nacc = be.to_numpy(na)
+    nrej = be.to_numpy(nr)
+    return {"reactors": n, "seconds": t, "reactor_solves_per_s": n / t, "steps_accepted_mean": float(nacc.mean()),
+            "steps_accepted_max": int(nacc.max()), "steps_rejected_total": int(nrej.sum()),
+            "integrator": "ROS4, central-difference Jacobian, FP64 LU (rosenbrock_kernel per reactor)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +156,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=80)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--reactors", type=int, default=0,
+                    help="also time the batched chemistry on this many reactors (e.g. 2097152)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (5 times): the known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -206,6 +238,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / rf.HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": dom_avg_s * 1e3, "algorithmic_bytes": dom_bytes},
     }
+    if args.reactors > 0:
+        print(f"[bench] batched chemistry, {args.reactors} reactors", file=sys.stderr, flush=True)
+        out["batched_chemistry"] = reactor_bench(pkg, args.reactors)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pkg, tables, args.cpu_n, args.cpu_steps)

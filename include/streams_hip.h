@@ -252,6 +252,25 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
                        int num_t_tab, double t_min_tab, double dt_tab, double r_univ,
                        double gamdt, const int *fluid_mask, void *stream);
 
+/* ---- batched chemistry ---------------------------------------------------------------- */
+/* nbatch independent constant-volume reactors, each integrated like one cell of
+ * rosenbrock_kernel (kernels_gpu.F90:8928-9193, launched by rosenbrock,
+ * multideal_gpu.F90:694-767) from t = 0 to time_end[b] with first trial step dttry[b].
+ * State is SoA on the device: w[m*nbatch + b] (m = 0..N_S+3, the conserved variables of
+ * w(i,j,k,1:nv)), tguess[b] in (Newton start) and out (last temperature); nacc/nrej (may be
+ * NULL) receive the accepted / rejected step counts.  Kinetics come from the device copy of
+ * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
+ * counter (zeroed by the call on the stream); wg_per_cu <= 0 sizes the persistent grid to
+ * the resident capacity.  No counterpart as one routine in the reference: its
+ * rosenbrock_kernel takes the cells of a field, this takes a flat batch. */
+int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
+                          const double *time_end, const double *mw, int nreactions,
+                          const double *kin_pack, const double *kc_tab, int num_t_tab,
+                          double t_min_tab, double dt_tab, double r_univ, double tol_iter_nr,
+                          const double *cv_coeff, int nsetcv, const double *trange, int indx_cp_l,
+                          int indx_cp_r, int maxsteps, int maxtry, double tol, int *nacc, int *nrej,
+                          unsigned *work, int wg_per_cu, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

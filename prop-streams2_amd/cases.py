@@ -98,18 +98,18 @@ def turbulence_box(tables: Tables, n: int, seed: int = 2024, ng: int = 3, temper
 
 def reactor_batch(tables: Tables, n: int, seed: int = 7, t_range=(800.0, 2500.0),
                   p_range=(0.5 * 101325.0, 20 * 101325.0), alpha: float = 1.0):
-    """n independent reactors: SoA conserved state w[nv, n] and the T guess."""
+    """n independent reactors at rest: SoA conserved state w[nv, n] and the T guess; T and p
+    log-uniform on the given ranges, Y from a seeded Dirichlet(alpha) over the species."""
     rng = np.random.default_rng(seed)
     ns = tables.n_species
     T = np.exp(rng.uniform(math.log(t_range[0]), math.log(t_range[1]), n))
     p = np.exp(rng.uniform(math.log(p_range[0]), math.log(p_range[1]), n))
     Y = rng.dirichlet(np.full(ns, alpha), size=n)
+    rmix = Y @ tables.rgas
+    rho = p / (rmix * T)
     w = np.zeros((ns + 4, n))
-    for b in range(n):
-        y = Y[b]
-        rho = p[b] / (tables.rmix(y) * T[b])
-        w[:ns, b] = rho * y
-        w[ns + 3, b] = rho * tables.e_from_T(T[b], y)
+    w[:ns] = (rho[:, None] * Y).T
+    w[ns + 3] = rho * tables.e_from_T_many(T, Y)
     return w, T.copy(), dict(T=T, p=p, Y=Y)
 
 

@@ -201,11 +201,11 @@ struct RosStats {
   int nacc = 0, nrej = 0;
 };
 
-// One cell from time_start to time_end (rosenbrock_kernel body)
-template <int NS>
-__device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_nr, double *wv, double &tg,
-                                double dttry, double time_start, double time_end, int maxsteps, int maxtry,
-                                double tol, RosStats &st) {
+// One cell from time_start to time_end (rosenbrock_kernel body); rhs(wv, tg, out) is the
+// chemistry right-hand side of rosenbrock_rhs (it may clip wv's species and updates tg).
+template <int NS, class Rhs>
+__device__ void rosenbrock_cell_f(const Rhs &rhs_fn, double *wv, double &tg, double dttry, double time_start,
+                                  double time_end, int maxsteps, int maxtry, double tol, RosStats &st) {
   constexpr int NV = NS + 4;
   const double safety = 0.9, grow = 1.5, pgrow = -0.25, shrnk = 0.5, pshrnk = -1.0 / 3.0, errcon = 0.1296;
   const double gam = 1.0 / 2.0, a21 = 2.0, a31 = 48.0 / 25.0, a32 = 6.0 / 25.0;
@@ -225,13 +225,13 @@ __device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_
       const double wtmp = wv[n];
       const double h = wtmp * 1.0e-06 + 1.0e-14;
       wv[n] = wtmp + h;
-      ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, rp);
+      rhs_fn(wv, tg, rp);
       wv[n] = wtmp - h;
-      ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, rm);
+      rhs_fn(wv, tg, rm);
       wv[n] = wtmp;
       for (int m = 0; m < NV; ++m) jac[m + NV * n] = (rp[m] - rm[m]) / (2.0 * h);
     }
-    ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, rhs_sav);
+    rhs_fn(wv, tg, rhs_sav);
     for (int m = 0; m < NV; ++m) {
       w_sav[m] = wv[m];
       yscal[m] = fmax(1.0, fabs(w_sav[m]));
@@ -246,12 +246,12 @@ __device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_
       lubksb<NV>(amat, indx, g1);
       for (int m = 0; m < NV; ++m) wv[m] = w_sav[m] + a21 * g1[m];
       time = t0 + dt;
-      ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, rhs);
+      rhs_fn(wv, tg, rhs);
       for (int m = 0; m < NV; ++m) g2[m] = rhs[m] + c21 * g1[m] / dt;
       lubksb<NV>(amat, indx, g2);
       for (int m = 0; m < NV; ++m) wv[m] = w_sav[m] + a31 * g1[m] + a32 * g2[m];
       time = t0 + 3.0 / 5.0 * dt;
-      ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, rhs);
+      rhs_fn(wv, tg, rhs);
       for (int m = 0; m < NV; ++m) g3[m] = rhs[m] + (c31 * g1[m] + c32 * g2[m]) / dt;
       lubksb<NV>(amat, indx, g3);
       for (int m = 0; m < NV; ++m) g4[m] = rhs[m] + (c41 * g1[m] + c42 * g2[m] + c43 * g3[m]) / dt;
@@ -280,6 +280,25 @@ __device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_
     if (dtnext * 1.5 > remaining) { dt = remaining; finish = 1; }
     else { dt = dtnext; finish = 0; }
   }
+}
+
+// The reference-form right-hand side (rosenbrock_rhs) bound to its tables.
+template <int NS>
+struct RosRhsRef {
+  const Kin &K;
+  const Thermo &th;
+  double tol_iter_nr;
+  __device__ inline void operator()(double *wv, double &tg, double *out) const {
+    ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, out);
+  }
+};
+
+template <int NS>
+__device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_nr, double *wv, double &tg,
+                                double dttry, double time_start, double time_end, int maxsteps, int maxtry,
+                                double tol, RosStats &st) {
+  const RosRhsRef<NS> f{K, th, tol_iter_nr};
+  rosenbrock_cell_f<NS>(f, wv, tg, dttry, time_start, time_end, maxsteps, maxtry, tol, st);
 }
 
 }  // namespace streams

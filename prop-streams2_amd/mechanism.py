@@ -561,6 +561,29 @@ class Tables:
             ee += cvl * tprod
         return ee
 
+    def e_from_T_many(self, T: np.ndarray, Y: np.ndarray) -> np.ndarray:
+        """e_from_T for many states at once: T (n,), Y (n, N_S); the same sums in the same
+        order, vectorised over the states."""
+        lo, hi = self.indx_cp_l, self.indx_cp_r
+        T = np.asarray(T, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        ns = self.n_species
+        nr = np.ones((T.size, ns), dtype=np.int64)
+        for m in range(2, self.nsetcv + 1):
+            nr = np.where(T[:, None] >= self.trange[None, :, m - 1], m, nr)
+        cols = np.arange(ns)[None, :]
+        ee = np.zeros(T.size)
+        for k in range(ns):
+            ee = ee + self.cv_coeff[hi + 1 - lo, k, nr[:, k] - 1] * Y[:, k]
+        for l in range(lo, hi + 1):
+            tprod = np.log(T) if l == -1 else T ** (l + 1) / (l + 1.0)
+            cvl = np.zeros(T.size)
+            for k in range(ns):
+                cvl = cvl + self.cv_coeff[l - lo, k, nr[:, k] - 1] * Y[:, k]
+            ee = ee + cvl * tprod
+        del cols
+        return ee
+
     def cp_from_T(self, T: float, Y: Sequence[float]) -> float:
         """Mixture c_p (get_cp, multideal.F90:5720-5761)."""
         lo, hi = self.indx_cp_l, self.indx_cp_r

@@ -38,9 +38,10 @@ def test_binding_table_matches_header(pkg):
 
 def test_oracle_exports_same_entry_points(pkg, oracle):
     """Every entry point that restates one reference routine has its CPU twin (the fused
-    rhs_* stages are checked against sequences of those twins)."""
+    rhs_* stages are checked against sequences of those twins, the reactor batch against the
+    oracle's per-reactor Rosenbrock)."""
     for name in pkg.capi.SIGNATURES:
-        if name.startswith("rhs_"):
+        if name.startswith("rhs_") or name == "reactor_batch":
             continue
         assert oracle.has(name), name
 
