@@ -119,7 +119,8 @@ def copy_probe(gib: float):
 
 def reactor_bench(pkg, n: int, reps: int = 3):
     """Batched constant-volume reactors (T log-uniform 800-2500 K, p 0.5-20 atm, Dirichlet
-    Y, horizon 1e-6 s): reactor-solves/s of the plain FP64 Rosenbrock batch."""
+    Y, horizon 1e-6 s): reactor-solves/s of the FP64 Rosenbrock batch and of its
+    mixed-precision variant, and the largest relative difference in T between the two."""
     import torch
     tables = pkg.cases.h2_air_mechanism().tables()
     w, tg, dttry, tend, info = pkg.reactors.reactor_inputs(tables, n)
@@ -127,24 +128,30 @@ def reactor_bench(pkg, n: int, reps: int = 3):
     G = pkg.reactors.ReactorBatchGPU(be, tables, tol=1e-8)
     d_dt, d_end = be.asarray(dttry), be.asarray(tend)
     na, nr = be.zeros((n,), "i4"), be.zeros((n,), "i4")
-    times = []
-    for rep in range(reps + 1):
-        wd, td = be.asarray(w), be.asarray(tg)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        G.integrate(wd, td, d_dt, d_end, na, nr)
-        e1.record()
-        torch.cuda.synchronize()
-        if rep > 0:
-            times.append(e0.elapsed_time(e1) / 1e3)
-    times.sort()
-    t = times[len(times) // 2]
-    nacc = be.to_numpy(na)
-    nrej = be.to_numpy(nr)
-    return {"reactors": n, "seconds": t, "reactor_solves_per_s": n / t, "steps_accepted_mean": float(nacc.mean()),
-            "steps_accepted_max": int(nacc.max()), "steps_rejected_total": int(nrej.sum()),
-            "integrator": "ROS4, central-difference Jacobian, FP64 LU (rosenbrock_kernel per reactor)"}
+    out = {"reactors": n, "horizon_s": float(tend[0]), "tol": 1e-8}
+    finals = {}
+    for mixed in (False, True):
+        times = []
+        for rep in range(reps + 1):
+            wd, td = be.asarray(w), be.asarray(tg)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            G.integrate(wd, td, d_dt, d_end, na, nr, mixed=mixed)
+            e1.record()
+            torch.cuda.synchronize()
+            if rep > 0:
+                times.append(e0.elapsed_time(e1) / 1e3)
+        times.sort()
+        t = times[len(times) // 2]
+        nacc = be.to_numpy(na)
+        key = "mixed" if mixed else "fp64"
+        finals[key] = be.to_numpy(td)
+        out[key] = {"seconds": t, "reactor_solves_per_s": n / t, "steps_accepted_mean": float(nacc.mean()),
+                    "steps_accepted_max": int(nacc.max()), "steps_rejected_total": int(be.to_numpy(nr).sum())}
+    out["mixed_speedup"] = out["mixed"]["reactor_solves_per_s"] / out["fp64"]["reactor_solves_per_s"]
+    out["max_rel_diff_T_last_rhs"] = float(abs(finals["mixed"] - finals["fp64"]).max() / finals["fp64"].min())
+    return out
 
 
 def main():

@@ -261,15 +261,18 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
  * NULL) receive the accepted / rejected step counts.  Kinetics come from the device copy of
  * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
  * counter (zeroed by the call on the stream); wg_per_cu <= 0 sizes the persistent grid to
- * the resident capacity.  No counterpart as one routine in the reference: its
- * rosenbrock_kernel takes the cells of a field, this takes a flat batch. */
+ * the resident capacity.  mixed = 0 is the FP64 integrator; mixed = 1 factors the iteration
+ * matrix in FP32, refines every stage solve once against the FP64 residual and forms the
+ * Jacobian by one-sided differences (same stages, error norm and step control).  No
+ * counterpart as one routine in the reference: its rosenbrock_kernel takes the cells of a
+ * field, this takes a flat batch. */
 int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
                           const double *time_end, const double *mw, int nreactions,
                           const double *kin_pack, const double *kc_tab, int num_t_tab,
                           double t_min_tab, double dt_tab, double r_univ, double tol_iter_nr,
                           const double *cv_coeff, int nsetcv, const double *trange, int indx_cp_l,
                           int indx_cp_r, int maxsteps, int maxtry, double tol, int *nacc, int *nrej,
-                          unsigned *work, int wg_per_cu, void *stream);
+                          unsigned *work, int wg_per_cu, int mixed, void *stream);
 
 #ifdef __cplusplus
 }

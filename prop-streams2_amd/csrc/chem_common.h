@@ -134,63 +134,63 @@ __device__ void ros_rhs(const Kin &K, const Thermo &th, double tol_iter_nr, doub
 }
 
 // ludcmp: Crout LU with implicit partial pivoting, a is n x n column-major
-template <int N>
-__device__ void ludcmp(double *a, int *indx) {
-  const double tiny = 1.5e-16;
-  double vv[N];
+template <int N, class T = double>
+__device__ void ludcmp(T *a, int *indx) {
+  const T tiny = (T)1.5e-16;
+  T vv[N];
 #define A_(r, c) a[(r) + N * (c)]
   for (int i = 0; i < N; ++i) {
-    double amax = 0.0;
+    T amax = 0;
     for (int j = 0; j < N; ++j) amax = fabs(A_(i, j)) > amax ? fabs(A_(i, j)) : amax;
     if (amax < tiny) return;
-    vv[i] = 1.0 / amax;
+    vv[i] = (T)1 / amax;
   }
   int imax = 0;
   for (int j = 0; j < N; ++j) {
     for (int i = 0; i < j; ++i) {
-      double summ = A_(i, j);
+      T summ = A_(i, j);
       for (int k = 0; k < i; ++k) summ = summ - A_(i, k) * A_(k, j);
       A_(i, j) = summ;
     }
-    double amax = 0.0;
+    T amax = 0;
     for (int i = j; i < N; ++i) {
-      double summ = A_(i, j);
+      T summ = A_(i, j);
       for (int k = 0; k < j; ++k) summ = summ - A_(i, k) * A_(k, j);
       A_(i, j) = summ;
-      double dum = vv[i] * fabs(summ);
+      T dum = vv[i] * fabs(summ);
       if (dum >= amax) { imax = i; amax = dum; }
     }
     if (j != imax) {
-      for (int k = 0; k < N; ++k) { double d = A_(imax, k); A_(imax, k) = A_(j, k); A_(j, k) = d; }
+      for (int k = 0; k < N; ++k) { T d = A_(imax, k); A_(imax, k) = A_(j, k); A_(j, k) = d; }
       vv[imax] = vv[j];
     }
     indx[j] = imax;
     if (fabs(A_(j, j)) < tiny) A_(j, j) = tiny;
     if (j != N - 1) {
-      double dum = 1.0 / A_(j, j);
+      T dum = (T)1 / A_(j, j);
       for (int i = j + 1; i < N; ++i) A_(i, j) = A_(i, j) * dum;
     }
   }
 #undef A_
 }
 
-template <int N>
-__device__ void lubksb(const double *a, const int *indx, double *bv) {
+template <int N, class T = double>
+__device__ void lubksb(const T *a, const int *indx, T *bv) {
 #define A_(r, c) a[(r) + N * (c)]
   int ii = -1;
   for (int i = 0; i < N; ++i) {
     int ll = indx[i];
-    double summ = bv[ll];
+    T summ = bv[ll];
     bv[ll] = bv[i];
     if (ii >= 0) {
       for (int j = ii; j < i; ++j) summ = summ - A_(i, j) * bv[j];
-    } else if (summ != 0.0) {
+    } else if (summ != (T)0) {
       ii = i;
     }
     bv[i] = summ;
   }
   for (int i = N - 1; i >= 0; --i) {
-    double summ = bv[i];
+    T summ = bv[i];
     for (int j = i + 1; j < N; ++j) summ = summ - A_(i, j) * bv[j];
     bv[i] = summ / A_(i, i);
   }

@@ -50,19 +50,130 @@ struct RosRhsPacked {
   }
 };
 
+// Mixed-precision variant of the same step (same ROS4 stages, step-size control and error
+// norm): the iteration matrix W = I/(gamma h) - J is factored in FP32 and every stage solve
+// is refined once against the FP64 residual b - W x, with J kept in FP64.  The Jacobian is
+// a one-sided difference (nv + 1 right-hand sides instead of 2 nv + 1): its O(h) error is
+// of the order of the FP32 factorisation's own rounding, which the FP64 refinement
+// removes from the stage solutions.  The right-hand sides, the temperature inversion and
+// the error estimate stay FP64.
+template <int NV>
+__device__ inline void solve_refined(const double *jac, const float *lu, const int *indx, double gi, double *b) {
+  float x32[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) x32[m] = (float)b[m];
+  lubksb<NV, float>(lu, indx, x32);
+  double x[NV], r[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) x[m] = (double)x32[m];
+  // r = b - W x with W = gi*I - J (FP64)
+#pragma unroll
+  for (int m = 0; m < NV; ++m) r[m] = b[m] - gi * x[m];
+  for (int n = 0; n < NV; ++n) {
+    const double xn = x[n];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) r[m] = r[m] + jac[m + NV * n] * xn;
+  }
+#pragma unroll
+  for (int m = 0; m < NV; ++m) x32[m] = (float)r[m];
+  lubksb<NV, float>(lu, indx, x32);
+#pragma unroll
+  for (int m = 0; m < NV; ++m) b[m] = x[m] + (double)x32[m];
+}
+
+template <int NS, class Rhs>
+__device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, double dttry, double time_start,
+                                   double time_end, int maxsteps, int maxtry, double tol, RosStats &st) {
+  constexpr int NV = NS + 4;
+  const double safety = 0.9, grow = 1.5, pgrow = -0.25, shrnk = 0.5, pshrnk = -1.0 / 3.0, errcon = 0.1296;
+  const double gam = 1.0 / 2.0, a21 = 2.0, a31 = 48.0 / 25.0, a32 = 6.0 / 25.0;
+  const double c21 = -8.0, c31 = 372.0 / 25.0, c32 = 12.0 / 5.0;
+  const double c41 = -112.0 / 125.0, c42 = -54.0 / 125.0, c43 = -2.0 / 5.0;
+  const double b1 = 19.0 / 9.0, b2 = 1.0 / 2.0, b3 = 25.0 / 108.0, b4 = 125.0 / 108.0;
+  const double e1 = 17.0 / 54.0, e2 = 7.0 / 36.0, e3 = 0.0, e4 = 125.0 / 108.0;
+  double jac[NV * NV];
+  float lu[NV * NV];
+  double w_sav[NV], rhs_sav[NV], yscal[NV], rhs[NV], rp[NV];
+  double g1[NV], g2[NV], g3[NV], g4[NV];
+  int indx[NV];
+  int finish = 0;
+  double time = time_start, dt = dttry, itol = 1.0 / tol, dtnext = dt;
+  for (int iter = 1; iter <= maxsteps; ++iter) {
+    const double t0 = time;
+    rhs_fn(wv, tg, rhs_sav);
+    for (int n = 0; n < NV; ++n) {
+      const double wtmp = wv[n];
+      const double h = wtmp * 1.0e-06 + 1.0e-14;
+      wv[n] = wtmp + h;
+      double tj = tg;
+      rhs_fn(wv, tj, rp);
+      wv[n] = wtmp;
+      for (int m = 0; m < NV; ++m) jac[m + NV * n] = (rp[m] - rhs_sav[m]) / h;
+    }
+    for (int m = 0; m < NV; ++m) {
+      w_sav[m] = wv[m];
+      yscal[m] = fmax(1.0, fabs(w_sav[m]));
+    }
+    for (int jtry = 1; jtry <= maxtry; ++jtry) {
+      const double gi = 1.0 / (gam * dt);
+      for (int n = 0; n < NV; ++n) {
+        for (int m = 0; m < NV; ++m) lu[m + NV * n] = (float)(-jac[m + NV * n]);
+        lu[n + NV * n] = (float)(gi - jac[n + NV * n]);
+      }
+      ludcmp<NV, float>(lu, indx);
+      for (int m = 0; m < NV; ++m) g1[m] = rhs_sav[m];
+      solve_refined<NV>(jac, lu, indx, gi, g1);
+      for (int m = 0; m < NV; ++m) wv[m] = w_sav[m] + a21 * g1[m];
+      time = t0 + dt;
+      rhs_fn(wv, tg, rhs);
+      for (int m = 0; m < NV; ++m) g2[m] = rhs[m] + c21 * g1[m] / dt;
+      solve_refined<NV>(jac, lu, indx, gi, g2);
+      for (int m = 0; m < NV; ++m) wv[m] = w_sav[m] + a31 * g1[m] + a32 * g2[m];
+      time = t0 + 3.0 / 5.0 * dt;
+      rhs_fn(wv, tg, rhs);
+      for (int m = 0; m < NV; ++m) g3[m] = rhs[m] + (c31 * g1[m] + c32 * g2[m]) / dt;
+      solve_refined<NV>(jac, lu, indx, gi, g3);
+      for (int m = 0; m < NV; ++m) g4[m] = rhs[m] + (c41 * g1[m] + c42 * g2[m] + c43 * g3[m]) / dt;
+      solve_refined<NV>(jac, lu, indx, gi, g4);
+      double errmax = 0.0;
+      for (int m = 0; m < NV; ++m) {
+        wv[m] = w_sav[m] + b1 * g1[m] + b2 * g2[m] + b3 * g3[m] + b4 * g4[m];
+        const double err = e1 * g1[m] + e2 * g2[m] + e3 * g3[m] + e4 * g4[m];
+        errmax = fmax(errmax, fabs(err / yscal[m]));
+      }
+      time = t0 + dt;
+      errmax = errmax * itol;
+      if (errmax < 1.0 || finish == 1) {
+        if (errmax > errcon) dtnext = safety * dt * pow(errmax, pgrow);
+        else dtnext = grow * dt;
+        break;
+      } else {
+        st.nrej += 1;
+        dtnext = safety * dt * pow(errmax, pshrnk);
+        dt = copysign(fmax(fabs(dtnext), shrnk * fabs(dt)), dt);
+      }
+    }
+    st.nacc += 1;
+    const double remaining = time_end - time;
+    if (finish == 1 || remaining < 1.0e-12 * time_end) break;
+    if (dtnext * 1.5 > remaining) { dt = remaining; finish = 1; }
+    else { dt = dtnext; finish = 0; }
+  }
+}
+
 struct ReactorArgs {
   double *w, *tg;                 // w[m * nbatch + b] (SoA), T guess / result
   const double *dttry, *time_end; // per reactor
   int *nacc, *nrej;               // per reactor (may be null)
   unsigned *next;                 // work counter (starts at 0)
-  int nbatch, maxsteps, maxtry;
+  int nbatch, maxsteps, maxtry, mixed;
   double tol, tol_iter_nr;
   const double *mw;
   KinPacked K;
   Thermo th;
 };
 
-template <int NS, int LO>
+template <int NS, int LO, bool MIXED>
 __global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
   constexpr int NV = NS + 4;
   __shared__ double tlds[thermo_lds_doubles<NS>(TH_CV | TH_CVH)];
@@ -78,7 +189,8 @@ __global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
     for (int m = 0; m < NV; ++m) wv[m] = A.w[(long)m * nb + r];
     double tg = A.tg[r];
     RosStats st;
-    rosenbrock_cell_f<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+    if (MIXED) rosenbrock_cell_mp<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+    else rosenbrock_cell_f<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
 #pragma unroll
     for (int m = 0; m < NV; ++m) A.w[(long)m * nb + r] = wv[m];
     A.tg[r] = tg;
@@ -90,22 +202,28 @@ __global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
 __global__ void zero_counter_kernel(unsigned *c) { *c = 0u; }
 
 // resident workgroups of reactor_kernel<NS, LO> on this device (persistent grid size)
-template <int NS, int LO>
+template <int NS, int LO, bool MIXED>
 int reactor_grid(int wg_per_cu) {
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reactor_kernel<NS, LO>, 256, 0) != hipSuccess) occ = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reactor_kernel<NS, LO, MIXED>, 256, 0) != hipSuccess) occ = 1;
   if (wg_per_cu > 0 && wg_per_cu < occ) occ = wg_per_cu;
   return ncu * (occ > 0 ? occ : 1);
 }
 
 template <int NS, int LO>
 int reactor_launch(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
-  const int g = reactor_grid<NS, LO>(wg_per_cu);
-  if (g <= 0) return STREAMS_E_DIMS;
-  reactor_kernel<NS, LO><<<g, 256, 0, st>>>(A);
+  if (A.mixed) {
+    const int g = reactor_grid<NS, LO, true>(wg_per_cu);
+    if (g <= 0) return STREAMS_E_DIMS;
+    reactor_kernel<NS, LO, true><<<g, 256, 0, st>>>(A);
+  } else {
+    const int g = reactor_grid<NS, LO, false>(wg_per_cu);
+    if (g <= 0) return STREAMS_E_DIMS;
+    reactor_kernel<NS, LO, false><<<g, 256, 0, st>>>(A);
+  }
   return hip_status();
 }
 
@@ -114,7 +232,7 @@ int reactor_launch(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
 using namespace streams;
 
 // Batched integration of nbatch independent constant-volume reactors (rosenbrock_kernel per
-// reactor) from t = 0 to time_end[b].  work points to one unsigned of device memory used as
+// reactor, or its mixed-precision variant when mixed != 0) from t = 0 to time_end[b].  work points to one unsigned of device memory used as
 // the shared work counter (zeroed here, on the stream); wg_per_cu <= 0 lets the occupancy
 // calculator size the persistent grid.
 extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
@@ -123,14 +241,14 @@ extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tgu
                                      double dt_tab, double r_univ, double tol_iter_nr, const double *cv_coeff,
                                      int nsetcv, const double *trange, int indx_cp_l, int indx_cp_r, int maxsteps,
                                      int maxtry, double tol, int *nacc, int *nrej, unsigned *work, int wg_per_cu,
-                                     void *stream) {
+                                     int mixed, void *stream) {
   if (nbatch < 0 || nreactions < 0 || maxsteps < 1 || maxtry < 1) return STREAMS_E_DIMS;
   if (n_s > KTB) return STREAMS_E_NSPECIES;
   if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;
   if (nbatch == 0) return 0;
   if (work == nullptr) return STREAMS_E_DIMS;
   hipStream_t st = as_stream(stream);
-  ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, work, nbatch, maxsteps, maxtry, tol, tol_iter_nr, mw,
+  ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, work, nbatch, maxsteps, maxtry, mixed != 0, tol, tol_iter_nr, mw,
                 KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                 Thermo{cv_coeff, nullptr, trange, nsetcv, indx_cp_l, indx_cp_r}};
   zero_counter_kernel<<<1, 1, 0, st>>>(work);

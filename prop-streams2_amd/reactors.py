@@ -35,15 +35,17 @@ class ReactorBatchGPU:
         self.t_trange = f(tables.trange)
         self.work = backend.zeros((2,), "i4")
 
-    def integrate(self, w, tg, dttry, time_end, nacc=None, nrej=None, wg_per_cu: int = 0, stream=None):
+    def integrate(self, w, tg, dttry, time_end, nacc=None, nrej=None, wg_per_cu: int = 0, stream=None,
+                  mixed: bool = False):
         """w (nv, nb) float64, tg (nb) float64, dttry/time_end (nb) float64, all on the device;
-        nacc/nrej optional int32 (nb).  Enqueued on `stream` (default: the backend's)."""
+        nacc/nrej optional int32 (nb).  Enqueued on `stream` (default: the backend's).
+        mixed=True: FP32-factored iteration matrix with FP64 residual refinement."""
         tb = self.tb
         nb = int(w.shape[1])
         self.be.call("reactor_batch", self.ns, nb, w, tg, dttry, time_end, self.t_mw, tb.nreactions, self.t_kin,
                      self.t_kc, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, self.t_cv, tb.nsetcv,
                      self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.maxsteps, self.maxtry, self.tol, nacc, nrej,
-                     self.work, wg_per_cu, self.be.stream if stream is None else stream)
+                     self.work, wg_per_cu, int(mixed), self.be.stream if stream is None else stream)
 
 
 def reactor_inputs(tables: Tables, n: int, seed: int = 7, dt: float = 1e-6):

@@ -90,3 +90,57 @@ def test_reactor_batch_ignition_against_cantera(pkg, hip):
         rho_n = wv[:tables.n_species].sum()
         temps[row] = reactor_util.t_from_e(tables, wv[-1] / rho_n, wv[:tables.n_species] / rho_n)
     assert np.abs(temps - gold[:, 2]).max() < 1.0
+
+
+def _ignition_histories(pkg, hip, tables, n, mixed, n_out=200, t_out=5e-6, seed=11):
+    """Stoichiometric H2-air reactors at 1000-1400 K, 1-10 atm, integrated over n_out
+    output intervals; returns T histories (n_out, n) from the final states of each
+    interval (host inversion of e)."""
+    rng = np.random.default_rng(seed)
+    ns = tables.n_species
+    names = [nm.upper() for nm in tables.names]
+    x = np.zeros(ns)
+    x[names.index("H2")], x[names.index("O2")], x[names.index("N2")] = 2.0, 1.0, 3.76
+    y = x * tables.mw / (x * tables.mw).sum()
+    T0 = rng.uniform(1000.0, 1400.0, n)
+    p0 = rng.uniform(1.0, 10.0, n) * 101325.0
+    Y = np.repeat(y[None, :], n, 0)
+    rho = p0 / ((Y @ tables.rgas) * T0)
+    w = np.zeros((ns + 4, n))
+    w[:ns] = (rho[:, None] * Y).T
+    w[ns + 3] = rho * tables.e_from_T_many(T0, Y)
+    G = pkg.reactors.ReactorBatchGPU(hip, tables, tol=1e-8)
+    wd, td = hip.asarray(w), hip.asarray(T0)
+    step = hip.asarray(np.full(n, t_out))
+    hist = np.empty((n_out, n))
+    for k in range(n_out):
+        G.integrate(wd, td, step, step, mixed=mixed)
+        wv = hip.to_numpy(wd)
+        r = wv[:ns].sum(axis=0)
+        hist[k] = [reactor_util.t_from_e(tables, wv[ns + 3, b] / r[b], wv[:ns, b] / r[b]) for b in range(n)]
+    return T0, hist
+
+
+def test_mixed_precision_matches_fp64_ignition(pkg, hip):
+    """Mixed-precision integrator vs the FP64 one: final T and ignition delay (first
+    crossing of T0 + 400 K, interpolated between outputs) within 1e-6 relative."""
+    tables = pkg.cases.h2_air_mechanism().tables()
+    n = 64
+    T0, h64 = _ignition_histories(pkg, hip, tables, n, mixed=False)
+    _, hmp = _ignition_histories(pkg, hip, tables, n, mixed=True)
+    t_out = 5e-6
+    times = t_out * np.arange(1, h64.shape[0] + 1)
+
+    def delay(h, b):
+        thr = T0[b] + 400.0
+        k = int(np.argmax(h[:, b] > thr))
+        assert h[k, b] > thr, b
+        t1, t2 = (times[k - 1], times[k]) if k > 0 else (0.0, times[0])
+        y1 = h[k - 1, b] if k > 0 else T0[b]
+        return t1 + (thr - y1) * (t2 - t1) / (h[k, b] - y1)
+
+    ignited = [b for b in range(n) if h64[-1, b] > T0[b] + 400.0]
+    assert len(ignited) >= n // 2
+    for b in ignited:
+        assert abs(delay(hmp, b) - delay(h64, b)) <= 1e-6 * delay(h64, b), b
+    assert (np.abs(hmp[-1] - h64[-1]) <= 1e-6 * h64[-1]).all()
