@@ -152,7 +152,11 @@ def _rate(d: dict, order: float, conc: float, ea_fac: float):
 class Mechanism:
     """A parsed mechanism restricted to a species subset, with table builders."""
 
-    def __init__(self, path: str, species: Optional[Sequence[str]] = None, phase: int = 0):
+    def __init__(self, path: str, species: Optional[Sequence[str]] = None, phase: int = 0,
+                 reactions: bool = True):
+        """species: a subset of the phase's species (every kept reaction must use only
+        these; third-body efficiencies of dropped species are ignored).  reactions=False
+        keeps the thermo and transport of the subset only (an inert mixture)."""
         with open(path) as fh:
             doc = yaml.safe_load(fh)
         self.path = path
@@ -169,7 +173,7 @@ class Mechanism:
         idx = {n: i for i, n in enumerate(names)}
         self.reactions: List[Reaction] = []
         dropped = set(all_names) - set(names)
-        for rd in doc.get("reactions", []):
+        for rd in (doc.get("reactions", []) if reactions else []):
             rx = self._reaction(rd, conc, ea_fac)
             used = set(rx.reactants) | set(rx.products)
             if used & dropped:

@@ -460,3 +460,95 @@ def test_uniform_state_has_no_rhs(pkg, be):
     num = pkg.solver.Numerics(enable_chemistry=0)
     fl = _rhs_box(pkg, be, tb, wp, (8, 8, 8), [1e-3] * 3, num)
     assert np.abs(fl).max() == 0.0
+
+
+# ------------------------------------------------------------ shock tube, exact Riemann
+def _exact_riemann(rl, ul, pl, rr, ur, pr, gam, x, t, x0):
+    """Exact solution of the Riemann problem for an ideal gas (Toro, ch. 4): density
+    profile at time t."""
+    from scipy.optimize import brentq
+    cl, cr = np.sqrt(gam * pl / rl), np.sqrt(gam * pr / rr)
+
+    def f(p, rk, pk, ck):
+        if p > pk:
+            a, b = 2.0 / ((gam + 1) * rk), (gam - 1) / (gam + 1) * pk
+            return (p - pk) * np.sqrt(a / (p + b))
+        return 2 * ck / (gam - 1) * ((p / pk) ** ((gam - 1) / (2 * gam)) - 1)
+
+    ps = brentq(lambda p: f(p, rl, pl, cl) + f(p, rr, pr, cr) + ur - ul, 1e-8 * min(pl, pr), 10 * max(pl, pr))
+    us = 0.5 * (ul + ur) + 0.5 * (f(ps, rr, pr, cr) - f(ps, rl, pl, cl))
+    g1 = (gam - 1) / (gam + 1)
+    out = np.empty_like(x)
+    for q, xq in enumerate(x):
+        s = (xq - x0) / t
+        if s <= us:      # left of contact
+            if ps > pl:  # left shock
+                sl = ul - cl * np.sqrt((gam + 1) / (2 * gam) * ps / pl + (gam - 1) / (2 * gam))
+                out[q] = rl if s < sl else rl * (ps / pl + g1) / (g1 * ps / pl + 1)
+            else:        # left rarefaction
+                shl, stl = ul - cl, us - cl * (ps / pl) ** ((gam - 1) / (2 * gam))
+                if s < shl:
+                    out[q] = rl
+                elif s > stl:
+                    out[q] = rl * (ps / pl) ** (1 / gam)
+                else:
+                    out[q] = rl * (2 / (gam + 1) + (gam - 1) / ((gam + 1) * cl) * (ul - s)) ** (2 / (gam - 1))
+        else:
+            if ps > pr:  # right shock
+                sr = ur + cr * np.sqrt((gam + 1) / (2 * gam) * ps / pr + (gam - 1) / (2 * gam))
+                out[q] = rr if s > sr else rr * (ps / pr + g1) / (g1 * ps / pr + 1)
+            else:
+                shr, str_ = ur + cr, us + cr * (ps / pr) ** ((gam - 1) / (2 * gam))
+                if s > shr:
+                    out[q] = rr
+                elif s < str_:
+                    out[q] = rr * (ps / pr) ** (1 / gam)
+                else:
+                    out[q] = rr * (2 / (gam + 1) - (gam - 1) / ((gam + 1) * cr) * (ur - s)) ** (2 / (gam - 1))
+    return out
+
+
+@pytest.mark.parametrize("flux_splitting", [0, 1])
+def test_weno_shock_tube_against_exact_riemann(pkg, be, flux_splitting):
+    """Sod-type tube in argon (NASA7 c_p = 5R/2 exactly, gamma = 5/3) with the WENO5-Z
+    branch on every face (sensor threshold below the sensor's range), Roe characteristic
+    or Rusanov splitting, extrapolation at both ends: the density matches the exact
+    Riemann solution (L1 error of a few percent of the jump, shock and contact positions)."""
+    mech = pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / "h2_premix_11sp.yaml"), species=["Ar"], reactions=False)
+    tb = mech.tables()
+    n, length = 200, 1.0
+    num = pkg.solver.Numerics(enable_chemistry=0, sensor_threshold=-1.0, weno_scheme=3, weno_version=1,
+                              flux_splitting=flux_splitting)
+    x = (np.arange(n) + 0.5) * length / n
+    tl, pl, tr, pr = 400.0, 1.0e5, 320.0, 1.0e4
+    temps = np.where(x < 0.5, tl, tr)
+    press = np.where(x < 0.5, pl, pr)
+    y = np.ones((1, n))
+    w1 = np.zeros((5, n))
+    for i in range(n):
+        rho = press[i] / (tb.rgas[0] * temps[i])
+        w1[0, i] = rho
+        w1[4, i] = rho * tb.e_from_T(temps[i], [1.0])
+    wp = np.pad(w1, [(0, 0), (3, 3)], mode="edge")
+    wp = np.broadcast_to(wp[:, None, None, :], (5, 7, 7, n + 6)).copy()
+    s = pkg.solver.Solver(be, tb, n, 1, 1, 3, [length, 1.0, 1.0], bctags=(2, 2, 0, 0, 0, 0), numerics=num,
+                          rho0=w1[0].max(), u0=float(np.sqrt(5.0 / 3.0 * tb.rgas[0] * tl)))
+    s.set_state(wp, np.full(wp.shape[1:], 350.0))
+    s.compute_aux()
+    t, t_end = 0.0, 6.0e-4
+    while t < t_end - 1e-12:
+        s.compute_dt(0.5)
+        s.dt = min(s.dt, t_end - t)
+        s.rk_sync(fused=False)
+        t += s.dt
+    rho = s.w[0, 3, 3, 3:-3]
+    ref = _exact_riemann(pl / (tb.rgas[0] * tl), 0.0, pl, pr / (tb.rgas[0] * tr), 0.0, pr, 5.0 / 3.0, x, t, 0.5)
+    jump = ref.max() - ref.min()
+    l1 = np.abs(rho - ref).mean() / jump
+    assert l1 < 0.02, l1
+    # the shock: the rightmost cell above the pre-shock density by half the post-shock rise
+    post = ref[x > 0.5].max()
+    pre = ref[-1]
+    xs_num = x[np.nonzero(rho > pre + 0.5 * (post - pre))[0].max()]
+    xs_ref = x[np.nonzero(ref > pre + 0.5 * (post - pre))[0].max()]
+    assert abs(xs_num - xs_ref) <= 2 * length / n
