@@ -253,12 +253,16 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
                        double gamdt, const int *fluid_mask, void *stream);
 
 /* ---- batched chemistry ---------------------------------------------------------------- */
-/* nbatch independent constant-volume reactors, each integrated like one cell of
- * rosenbrock_kernel (kernels_gpu.F90:8928-9193, launched by rosenbrock,
- * multideal_gpu.F90:694-767) from t = 0 to time_end[b] with first trial step dttry[b].
- * State is SoA on the device: w[m*nbatch + b] (m = 0..N_S+3, the conserved variables of
- * w(i,j,k,1:nv)), tguess[b] in (Newton start) and out (last temperature); nacc/nrej (may be
- * NULL) receive the accepted / rejected step counts.  Kinetics come from the device copy of
+/* nbatch independent reactors, each integrated like one cell of rosenbrock_kernel
+ * (kernels_gpu.F90:8928-9193, launched by rosenbrock, multideal_gpu.F90:694-767) from t = 0
+ * to time_end[b] with first trial step dttry[b].  State is SoA on the device,
+ * w[m*nbatch + b] for m = 0..N_S+3:
+ *   pressure == NULL: constant volume, w = the conserved variables of w(i,j,k,1:nv);
+ *   pressure != NULL: constant pressure pressure[b] (adiabatic), w = (Y_1..Y_N_S, 0, 0, 0,
+ *     h) with h the mixture enthalpy; T from h by Newton on cp_coeff, rho = p/(R T)
+ *     (rgas and cp_coeff are then required).  The reference's reactor is constant volume.
+ * tguess[b] is the Newton start in and the last temperature out; nacc/nrej (may be NULL)
+ * receive the accepted / rejected step counts.  Kinetics come from the device copy of
  * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
  * counter (zeroed by the call on the stream); wg_per_cu <= 0 sizes the persistent grid to
  * the resident capacity.  mixed = 0 is the FP64 integrator; mixed = 1 factors the iteration
@@ -267,7 +271,8 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
  * counterpart as one routine in the reference: its rosenbrock_kernel takes the cells of a
  * field, this takes a flat batch. */
 int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
-                          const double *time_end, const double *mw, int nreactions,
+                          const double *time_end, const double *pressure, const double *mw,
+                          const double *rgas, const double *cp_coeff, int nreactions,
                           const double *kin_pack, const double *kc_tab, int num_t_tab,
                           double t_min_tab, double dt_tab, double r_univ, double tol_iter_nr,
                           const double *cv_coeff, int nsetcv, const double *trange, int indx_cp_l,

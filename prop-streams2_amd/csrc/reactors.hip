@@ -50,6 +50,48 @@ struct RosRhsPacked {
   }
 };
 
+// Constant-pressure adiabatic reactor (the serving workload; the reference's reactor is
+// constant-volume).  State: wv = (Y_1..Y_NS, 0, 0, 0, h) with h the mixture enthalpy,
+// constant in time; T from the Newton inversion of h on the cp tables, rho = p/(R T),
+// dY_s/dt = omega_s W_s / rho.
+template <int NS, int LO>
+struct RosRhsPressure {
+  const KinPacked &K;
+  const ThermoLds &TL;
+  const double *mw, *rgas;
+  double pres, tol_iter_nr;
+  __device__ inline void operator()(double *wv, double &tg, double *rhs) const {
+    double yy[NS], conc[NS], acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) wv[s] = wv[s] > 0.0 ? wv[s] : 0.0;
+    double ysum = wv[0];
+#pragma unroll
+    for (int s = 1; s < NS; ++s) ysum = ysum + wv[s];
+    const double yi = 1.0 / ysum;
+    double rmix = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      yy[s] = wv[s] * yi;
+      rmix = rmix + rgas[s] * yy[s];
+    }
+    int nrange[NS];
+    const double tt = temperature_newton_lds<NS, LO>(wv[NS + 3], tg, TL, TL.cp, TL.cph, tol_iter_nr, yy, nrange);
+    tg = tt;
+    const double rho = pres / (rmix * tt);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      conc[s] = rho * yy[s] / mw[s];
+      acc[s] = 0.0;
+    }
+    chemistry_source_packed<NS>(K, tt, conc, acc);
+    const double ri = 1.0 / rho;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rhs[s] = -acc[s] * ri;
+#pragma unroll
+    for (int m = NS; m < NS + 4; ++m) rhs[m] = 0.0;
+  }
+};
+
 // Mixed-precision variant of the same step (same ROS4 stages, step-size control and error
 // norm): the iteration matrix W = I/(gamma h) - J is factored in FP32 and every stage solve
 // is refined once against the FP64 residual b - W x, with J kept in FP64.  The Jacobian is
@@ -168,75 +210,93 @@ struct ReactorArgs {
   unsigned *next;                 // work counter (starts at 0)
   int nbatch, maxsteps, maxtry, mixed;
   double tol, tol_iter_nr;
-  const double *mw;
+  const double *mw, *rgas;
+  const double *pressure;         // per reactor; null: constant volume
   KinPacked K;
   Thermo th;
 };
 
-template <int NS, int LO, bool MIXED>
-__global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
+template <int NS, bool MIXED, class Rhs>
+__device__ inline void integrate_one(const Rhs &f, const ReactorArgs &A, unsigned r) {
   constexpr int NV = NS + 4;
-  __shared__ double tlds[thermo_lds_doubles<NS>(TH_CV | TH_CVH)];
-  const ThermoLds TL = thermo_stage<NS, LO>(A.th, tlds, TH_CV | TH_CVH);
+  const long nb = A.nbatch;
+  double wv[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) wv[m] = A.w[(long)m * nb + r];
+  double tg = A.tg[r];
+  RosStats st;
+  if (MIXED) rosenbrock_cell_mp<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+  else rosenbrock_cell_f<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+#pragma unroll
+  for (int m = 0; m < NV; ++m) A.w[(long)m * nb + r] = wv[m];
+  A.tg[r] = tg;
+  if (A.nacc) A.nacc[r] = st.nacc;
+  if (A.nrej) A.nrej[r] = st.nrej;
+}
+
+template <int NS, int LO, bool MIXED, bool CP>
+__global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
+  constexpr int WHICH = CP ? (TH_CP | TH_CPH) : (TH_CV | TH_CVH);
+  __shared__ double tlds[thermo_lds_doubles<NS>(WHICH)];
+  const ThermoLds TL = thermo_stage<NS, LO>(A.th, tlds, WHICH);
   __syncthreads();
-  const RosRhsPacked<NS, LO> f{A.K, TL, A.mw, A.tol_iter_nr};
   const long nb = A.nbatch;
   for (;;) {
     const unsigned r = atomicAdd(A.next, 1u);
     if ((long)r >= nb) break;
-    double wv[NV];
-#pragma unroll
-    for (int m = 0; m < NV; ++m) wv[m] = A.w[(long)m * nb + r];
-    double tg = A.tg[r];
-    RosStats st;
-    if (MIXED) rosenbrock_cell_mp<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
-    else rosenbrock_cell_f<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
-#pragma unroll
-    for (int m = 0; m < NV; ++m) A.w[(long)m * nb + r] = wv[m];
-    A.tg[r] = tg;
-    if (A.nacc) A.nacc[r] = st.nacc;
-    if (A.nrej) A.nrej[r] = st.nrej;
+    if (CP) {
+      const RosRhsPressure<NS, LO> f{A.K, TL, A.mw, A.rgas, A.pressure[r], A.tol_iter_nr};
+      integrate_one<NS, MIXED>(f, A, r);
+    } else {
+      const RosRhsPacked<NS, LO> f{A.K, TL, A.mw, A.tol_iter_nr};
+      integrate_one<NS, MIXED>(f, A, r);
+    }
   }
 }
 
 __global__ void zero_counter_kernel(unsigned *c) { *c = 0u; }
 
 // resident workgroups of reactor_kernel<NS, LO> on this device (persistent grid size)
-template <int NS, int LO, bool MIXED>
+template <int NS, int LO, bool MIXED, bool CP>
 int reactor_grid(int wg_per_cu) {
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reactor_kernel<NS, LO, MIXED>, 256, 0) != hipSuccess) occ = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reactor_kernel<NS, LO, MIXED, CP>, 256, 0) != hipSuccess) occ = 1;
   if (wg_per_cu > 0 && wg_per_cu < occ) occ = wg_per_cu;
   return ncu * (occ > 0 ? occ : 1);
 }
 
+template <int NS, int LO, bool MIXED, bool CP>
+int reactor_launch_t(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
+  const int g = reactor_grid<NS, LO, MIXED, CP>(wg_per_cu);
+  if (g <= 0) return STREAMS_E_DIMS;
+  reactor_kernel<NS, LO, MIXED, CP><<<g, 256, 0, st>>>(A);
+  return hip_status();
+}
+
 template <int NS, int LO>
 int reactor_launch(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
-  if (A.mixed) {
-    const int g = reactor_grid<NS, LO, true>(wg_per_cu);
-    if (g <= 0) return STREAMS_E_DIMS;
-    reactor_kernel<NS, LO, true><<<g, 256, 0, st>>>(A);
-  } else {
-    const int g = reactor_grid<NS, LO, false>(wg_per_cu);
-    if (g <= 0) return STREAMS_E_DIMS;
-    reactor_kernel<NS, LO, false><<<g, 256, 0, st>>>(A);
-  }
-  return hip_status();
+  const bool cp = A.pressure != nullptr;
+  if (A.mixed) return cp ? reactor_launch_t<NS, LO, true, true>(A, wg_per_cu, st)
+                         : reactor_launch_t<NS, LO, true, false>(A, wg_per_cu, st);
+  return cp ? reactor_launch_t<NS, LO, false, true>(A, wg_per_cu, st)
+            : reactor_launch_t<NS, LO, false, false>(A, wg_per_cu, st);
 }
 
 }  // namespace streams
 
 using namespace streams;
 
-// Batched integration of nbatch independent constant-volume reactors (rosenbrock_kernel per
-// reactor, or its mixed-precision variant when mixed != 0) from t = 0 to time_end[b].  work points to one unsigned of device memory used as
-// the shared work counter (zeroed here, on the stream); wg_per_cu <= 0 lets the occupancy
-// calculator size the persistent grid.
+// Batched integration of nbatch independent reactors (rosenbrock_kernel per reactor, or its
+// mixed-precision variant when mixed != 0) from t = 0 to time_end[b]: constant volume when
+// pressure is null, else constant pressure pressure[b] with the state (Y, 0, 0, 0, h).  work
+// points to one unsigned of device memory used as the shared work counter (zeroed here, on
+// the stream); wg_per_cu <= 0 lets the occupancy calculator size the persistent grid.
 extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
-                                     const double *time_end, const double *mw, int nreactions,
+                                     const double *time_end, const double *pressure, const double *mw,
+                                     const double *rgas, const double *cp_coeff, int nreactions,
                                      const double *kin_pack, const double *kc_tab, int num_t_tab, double t_min_tab,
                                      double dt_tab, double r_univ, double tol_iter_nr, const double *cv_coeff,
                                      int nsetcv, const double *trange, int indx_cp_l, int indx_cp_r, int maxsteps,
@@ -248,9 +308,10 @@ extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tgu
   if (nbatch == 0) return 0;
   if (work == nullptr) return STREAMS_E_DIMS;
   hipStream_t st = as_stream(stream);
+  if (pressure != nullptr && (rgas == nullptr || cp_coeff == nullptr)) return STREAMS_E_DIMS;
   ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, work, nbatch, maxsteps, maxtry, mixed != 0, tol, tol_iter_nr, mw,
-                KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
-                Thermo{cv_coeff, nullptr, trange, nsetcv, indx_cp_l, indx_cp_r}};
+                rgas, pressure, KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
+                Thermo{cv_coeff, cp_coeff, trange, nsetcv, indx_cp_l, indx_cp_r}};
   zero_counter_kernel<<<1, 1, 0, st>>>(work);
   STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r, return (reactor_launch<NS, LO>(A, wg_per_cu, st))));
   return 0;

@@ -141,9 +141,23 @@ __device__ inline double mixture_cp_lds(const ThermoLds &t, const int *nrange, c
 // per-species range tracking by one step per iteration from the locate at t_start, at
 // most 10 iterations, stop at |dT| < tol.  The mixture sums of the coefficients are rebuilt
 // only when a range changes.  nrange returns the ranges of the last iteration.
+// The same Newton iteration on any pair of staged tables: (cv, cv/(l+1)) inverts the
+// internal energy, (cp, cp/(l+1)) the enthalpy.
+template <int NS, int LO>
+__device__ inline double temperature_newton_lds(double ee, double t_start, const ThermoLds &t, const double *c_rows,
+                                                const double *ch_rows, double tol_iter_nr, const double *yy,
+                                                int *nrange);
+
 template <int NS, int LO>
 __device__ inline double temperature_from_e_lds(double ee, double t_start, const ThermoLds &t, double tol_iter_nr,
                                                 const double *yy, int *nrange) {
+  return temperature_newton_lds<NS, LO>(ee, t_start, t, t.cv, t.cvh, tol_iter_nr, yy, nrange);
+}
+
+template <int NS, int LO>
+__device__ inline double temperature_newton_lds(double ee, double t_start, const ThermoLds &t, const double *c_rows,
+                                                const double *ch_rows, double tol_iter_nr, const double *yy,
+                                                int *nrange) {
   using N = Nasa<LO>;
 #pragma unroll
   for (int s = 0; s < NS; ++s) nrange[s] = range_of<NS>(t, s, t_start);
@@ -169,15 +183,15 @@ __device__ inline double temperature_from_e_lds(double ee, double t_start, const
     if (dirty) {
       sumb = 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) sumb = sumb + t.cv[((nrange[s] - 1) * NS + s) * TSTRIDE + N::NL] * yy[s];
+      for (int s = 0; s < NS; ++s) sumb = sumb + c_rows[((nrange[s] - 1) * NS + s) * TSTRIDE + N::NL] * yy[s];
 #pragma unroll
       for (int q = 0; q < N::NL; ++q) {
         double a = 0.0, h = 0.0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           const int row = ((nrange[s] - 1) * NS + s) * TSTRIDE + q;
-          a = a + t.cv[row] * yy[s];
-          h = h + t.cvh[row] * yy[s];
+          a = a + c_rows[row] * yy[s];
+          h = h + ch_rows[row] * yy[s];
         }
         cvl[q] = a;
         cvh[q] = h;

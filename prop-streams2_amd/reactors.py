@@ -29,6 +29,8 @@ class ReactorBatchGPU:
         self.tol, self.maxsteps, self.maxtry = tol, maxsteps, maxtry
         f = lambda a: backend.asarray(np.asarray(a, dtype=np.float64).T)  # noqa: E731
         self.t_mw = backend.asarray(tables.mw)
+        self.t_rgas = backend.asarray(tables.rgas)
+        self.t_cp = f(tables.cp_coeff)
         self.t_kin = backend.asarray(capi.kinetics_pack(tables, backend.lib))
         self.t_kc = f(tables.kc_tab)
         self.t_cv = f(tables.cv_coeff)
@@ -36,16 +38,19 @@ class ReactorBatchGPU:
         self.work = backend.zeros((2,), "i4")
 
     def integrate(self, w, tg, dttry, time_end, nacc=None, nrej=None, wg_per_cu: int = 0, stream=None,
-                  mixed: bool = False):
+                  mixed: bool = False, pressure=None, work=None):
         """w (nv, nb) float64, tg (nb) float64, dttry/time_end (nb) float64, all on the device;
         nacc/nrej optional int32 (nb).  Enqueued on `stream` (default: the backend's).
-        mixed=True: FP32-factored iteration matrix with FP64 residual refinement."""
+        mixed=True: FP32-factored iteration matrix with FP64 residual refinement.
+        pressure (nb) on the device: constant-pressure reactors with w = (Y, 0, 0, 0, h).
+        work: a private counter for launches that may overlap on other streams."""
         tb = self.tb
         nb = int(w.shape[1])
-        self.be.call("reactor_batch", self.ns, nb, w, tg, dttry, time_end, self.t_mw, tb.nreactions, self.t_kin,
-                     self.t_kc, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, self.t_cv, tb.nsetcv,
-                     self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.maxsteps, self.maxtry, self.tol, nacc, nrej,
-                     self.work, wg_per_cu, int(mixed), self.be.stream if stream is None else stream)
+        self.be.call("reactor_batch", self.ns, nb, w, tg, dttry, time_end, pressure, self.t_mw, self.t_rgas,
+                     self.t_cp, tb.nreactions, self.t_kin, self.t_kc, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV,
+                     TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.maxsteps,
+                     self.maxtry, self.tol, nacc, nrej, self.work if work is None else work, wg_per_cu, int(mixed),
+                     self.be.stream if stream is None else stream)
 
 
 def reactor_inputs(tables: Tables, n: int, seed: int = 7, dt: float = 1e-6):
