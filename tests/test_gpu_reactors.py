@@ -27,19 +27,20 @@ def hip(pkg):
 def _run_both(pkg, hip, oracle, tables, n, seed, dt, t_range=(800.0, 2500.0)):
     w, tg, info = pkg.cases.reactor_batch(tables, n, seed=seed, t_range=t_range)
     dtt = np.full(n, dt)
+    first = np.full(n, pkg.reactors.ROS_DTTRY)
     # oracle
     wc = np.ascontiguousarray(w.copy())
     tc = tg.copy()
     R = reactor_util.ReactorBatch(oracle, tables, tol=1e-8, maxsteps=10000, maxtry=40)
-    na_c, nr_c = R.integrate(wc, tc, dtt, dtt)
+    na_c, nr_c = R.integrate(wc, tc, first, dtt)
     # GPU
     G = pkg.reactors.ReactorBatchGPU(hip, tables, tol=1e-8)
     wd, td = hip.asarray(w), hip.asarray(tg)
-    dd = hip.asarray(dtt)
+    dd, d1 = hip.asarray(dtt), hip.asarray(first)
     na = hip.zeros((n,), "i4")
     nr = hip.zeros((n,), "i4")
     print(f"[reactors] n={n} seed={seed} ns={tables.n_species}", flush=True)
-    G.integrate(wd, td, dd, dd, na, nr)
+    G.integrate(wd, td, d1, dd, na, nr)
     hip.sync()
     return (hip.to_numpy(wd), hip.to_numpy(td), hip.to_numpy(na), hip.to_numpy(nr)), (wc, tc, na_c, nr_c)
 
@@ -68,8 +69,11 @@ def _check(g, c, tables, tol):
     wg, tgg, na, nr = g
     wc, tc, na_c, nr_c = c
     fin_c = np.isfinite(wc).all(axis=0)
-    assert (np.isfinite(wg).all(axis=0) == fin_c).all()      # the same reactors fail, if any
-    assert fin_c.mean() > 0.95
+    # Random Dirichlet compositions with large radical fractions at up to 2500 K recombine
+    # to far beyond the tables' temperature range within the horizon; the Newton inversion
+    # then fails for both integrators on the same reactors.
+    assert (np.isfinite(wg).all(axis=0) == fin_c).all()
+    assert fin_c.mean() > 0.98
     worst = 0.0
     for m in range(wc.shape[0]):
         scale = np.abs(wc[m, fin_c]).max()
@@ -167,9 +171,10 @@ def _ignition_histories(pkg, hip, tables, n, mixed, n_out=200, t_out=5e-6, seed=
     G = pkg.reactors.ReactorBatchGPU(hip, tables, tol=1e-8)
     wd, td = hip.asarray(w), hip.asarray(T0)
     step = hip.asarray(np.full(n, t_out))
+    first = hip.asarray(np.full(n, 1e-8))
     hist = np.empty((n_out, n))
     for k in range(n_out):
-        G.integrate(wd, td, step, step, mixed=mixed)
+        G.integrate(wd, td, first, step, mixed=mixed)
         wv = hip.to_numpy(wd)
         r = wv[:ns].sum(axis=0)
         hist[k] = [reactor_util.t_from_e(tables, wv[ns + 3, b] / r[b], wv[:ns, b] / r[b]) for b in range(n)]
