@@ -162,6 +162,24 @@ def reactor_bench(pkg, n: int, reps: int = 3):
     return out
 
 
+def serving_bench(pkg, duration: float, rate: float = 200.0):
+    """Poisson stream (seeded, `rate` req/s for `duration` s) of constant-pressure reactor
+    requests, 1-4096 reactors each (log-uniform), H2-air (9 species) and H2/CO syngas
+    (Keromnes 14 species), T0 900-1800 K, phi 0.3-3 (log-uniform), horizon 1e-5..1e-2 s
+    (log-uniform), served with continuous batching; throughput and latency percentiles."""
+    mechs = {"h2": pkg.cases.h2_air_mechanism().tables(),
+             "syngas": pkg.mechanism.Mechanism(str(ROOT / "tests/golden/mechanisms/keromnes2013_14sp.yaml")).tables()}
+    fuels = {"h2": {"H2": 1.0}, "syngas": {"H2": 0.5, "CO": 0.5}}
+    reqs = pkg.serving.poisson_requests(mechs, rate=rate, duration=duration, seed=2024, fuels=fuels)
+    be = pkg.solver.HipBackend("cuda:0")
+    srv = pkg.serving.ReactorServer(be, mechs, hbm_bytes=8 << 30, slices=10, mixed=True)
+    st = srv.run(reqs, realtime=True, max_seconds=240)
+    st.pop("completed")
+    st.update({"arrival_rate_per_s": rate, "stream_s": duration, "submitted": len(reqs),
+               "submitted_reactors": int(sum(r.n for r in reqs))})
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +191,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--reactors", type=int, default=0,
                     help="also time the batched chemistry on this many reactors (e.g. 2097152)")
+    ap.add_argument("--serving", type=float, default=0.0,
+                    help="also run the reactor server on a Poisson stream of this many seconds")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (5 times): the known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -256,6 +276,9 @@ def main():
     if args.reactors > 0:
         print(f"[bench] batched chemistry, {args.reactors} reactors", file=sys.stderr, flush=True)
         out["batched_chemistry"] = reactor_bench(pkg, args.reactors)
+    if args.serving > 0:
+        print(f"[bench] reactor server, {args.serving} s stream", file=sys.stderr, flush=True)
+        out["serving"] = serving_bench(pkg, args.serving)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pkg, tables, args.cpu_n, args.cpu_steps)

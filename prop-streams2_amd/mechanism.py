@@ -588,6 +588,23 @@ class Tables:
         del cols
         return ee
 
+    def cp_from_T_many(self, T: np.ndarray, Y: np.ndarray) -> np.ndarray:
+        """cp_from_T for many states: T (n,), Y (n, N_S)."""
+        lo, hi = self.indx_cp_l, self.indx_cp_r
+        T = np.asarray(T, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        ns = self.n_species
+        nr = np.ones((T.size, ns), dtype=np.int64)
+        for m in range(2, self.nsetcv + 1):
+            nr = np.where(T[:, None] >= self.trange[None, :, m - 1], m, nr)
+        cp = np.zeros(T.size)
+        for l in range(lo, hi + 1):
+            cpl = np.zeros(T.size)
+            for k in range(ns):
+                cpl = cpl + self.cp_coeff[l - lo, k, nr[:, k] - 1] * Y[:, k]
+            cp = cp + cpl * T ** l
+        return cp
+
     def cp_from_T(self, T: float, Y: Sequence[float]) -> float:
         """Mixture c_p (get_cp, multideal.F90:5720-5761)."""
         lo, hi = self.indx_cp_l, self.indx_cp_r
