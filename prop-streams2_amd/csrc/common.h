@@ -199,6 +199,20 @@ __device__ double species_hcoef(double tt, const double *tab, const Thermo &th, 
   return hh;
 }
 
+// ------------------------------------------------------------ XCD-aware block order
+// The MI355X dispatches consecutive workgroups of a launch round-robin to its 8 XCDs, each
+// with its own L2.  Tiles that share cache lines (halo rows, the misaligned ends of a
+// padded row) reuse them only when they run on the same XCD at about the same time, so a
+// launch over nblk tiles uses a grid of xcd_grid(nblk) workgroups and maps its workgroup
+// to the tile xcd_tile(): XCD x takes the x-th contiguous range of tiles, in order.
+constexpr unsigned NXCD = 8;
+
+__host__ __device__ inline unsigned xcd_grid(unsigned nblk) { return (nblk + NXCD - 1) / NXCD * NXCD; }
+
+__device__ inline unsigned xcd_tile(unsigned b, unsigned grid) {
+  return (b % NXCD) * (grid / NXCD) + b / NXCD;   // a permutation of 0..grid-1
+}
+
 // ------------------------------------------------------------------ launch helpers
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -37,6 +37,8 @@ constexpr int KCH = 6;                      // LDS slots per staging round
 struct SweepArgs {
   EulerArgs E;
   ViscArgs V;
+  double *fln;
+  unsigned ncb, nseg, ntiles;   // column tiles, segments along the sweep, their product with the third extent
   double rhodt;
   int init;          // fold init_flux: fln = -rhodt*fl_old, fl = contribution
   int visc;          // add the viscous flux
@@ -160,10 +162,15 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
   const ThermoLds TL = thermo_stage<NS, LO>(A.V.th, tlds, TH_CPH);
   // tile coordinates
   const int us = E.fs + 1, ue = E.fe;
-  const int seg = DIR == 1 ? blockIdx.x : blockIdx.y;
+  // tile of this workgroup (XCD-aware order; tiles beyond the last are idle)
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= A.ntiles) return;
+  const unsigned nfast = DIR == 1 ? A.nseg : A.ncb, nmid = DIR == 1 ? A.ncb : A.nseg;
+  const unsigned bfast = tile % nfast, bmid = (tile / nfast) % nmid, bslow = tile / (nfast * nmid);
+  const int seg = DIR == 1 ? bfast : bmid;
   const int s0 = us + seg * (S1 - 1);
-  const int cbase = 1 + (DIR == 1 ? blockIdx.y : blockIdx.x) * C;
-  const int t2 = 1 + blockIdx.z;
+  const int cbase = 1 + (DIR == 1 ? bmid : bfast) * C;
+  const int t2 = 1 + bslow;
   const int ncol = DIR == 1 ? b.ny : b.nx;
   const int lmax_b = E.lmax_base;
   const int p0 = s0 - LMX;
@@ -353,7 +360,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
 }
 
 template <int NS, int DIR, int LO>
-int sweep_launch(const Box &b, const SweepArgs &A, hipStream_t st) {
+int sweep_launch(const Box &b, SweepArgs A, hipStream_t st) {
   constexpr int C = TileCfg<DIR>::C, S1 = TileCfg<DIR>::S1;
   const int ncell = A.E.fe - A.E.fs;
   if (ncell < 1) return 0;
@@ -361,9 +368,12 @@ int sweep_launch(const Box &b, const SweepArgs &A, hipStream_t st) {
   const int ncol = DIR == 1 ? b.ny : b.nx;
   const int ncb = (ncol + C - 1) / C;
   const int nt2 = DIR == 3 ? b.ny : b.nz;
-  dim3 grid = DIR == 1 ? dim3(nseg, ncb, nt2) : dim3(ncb, nseg, nt2);
-  sweep_kernel<NS, DIR, LO><<<grid, 256, 0, st>>>(b, A.E.w_aux, A.E.fl, A.V.fl == nullptr ? nullptr : A.V.fl,
-                                                  A.E.fhat, A);
+  const unsigned long ntiles = (unsigned long)nseg * ncb * nt2;
+  if (ntiles > 0xffffff00ul) return STREAMS_E_DIMS;
+  A.ncb = ncb;
+  A.nseg = nseg;
+  A.ntiles = (unsigned)ntiles;
+  sweep_kernel<NS, DIR, LO><<<xcd_grid(A.ntiles), 256, 0, st>>>(b, A.E.w_aux, A.E.fl, A.fln, A.E.fhat, A);
   return hip_status();
 }
 
@@ -395,7 +405,8 @@ extern "C" int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, i
                   force_zero_flux_min, force_zero_flux_max, weno_scheme, weno_version, flux_splitting,
                   sensor_threshold, tol_iter_nr, rho0, u0, 0, n, n};
   A.E.check_shock_flag = sensor_threshold < 1.0;
-  A.V = ViscArgs{w_aux, coord, rgas, fln, th, r_univ, n};
+  A.V = ViscArgs{w_aux, coord, rgas, fl, th, r_univ, n};
+  A.fln = fln;
   A.rhodt = rhodt;
   A.init = init;
   A.visc = visc;
