@@ -21,45 +21,71 @@ struct FinishArgs {
 };
 
 // One interior cell per thread, i fastest, so that every access of a wave is a coalesced
-// 512-byte segment of one variable plane.  The species part (chemistry, then the RK
-// update of the species equations) and the momentum/energy part (reduced viscous terms,
-// then their RK update) run one after the other to keep few values live at a time.
+// 512-byte segment of one variable plane.  The species equations (chemistry, then their RK
+// update) and the momentum / energy equations (reduced viscous terms, then their RK update)
+// touch disjoint data, so they run as two kernels: each holds fewer registers than the
+// pair would, which buys occupancy, and the pair moves exactly the bytes of one fused pass.
 template <int NS>
-__global__ void __launch_bounds__(256) finish_kernel(Box b, double *__restrict__ w, const double *__restrict__ wa,
-                                                     double *__restrict__ fl, double *__restrict__ fln,
-                                                     FinishArgs F) {
+__device__ inline bool cell_of(const Box &b, long t, int &i, int &j, int &k, long &pt) {
+  if (t >= b.fstride()) return false;
+  i = 1 + (int)(t % b.nx);
+  const long r = t / b.nx;
+  j = 1 + (int)(r % b.ny);
+  k = 1 + (int)(r / b.ny);
+  pt = b.p(i, j, k);
+  return true;
+}
+
+// Hides the value of x from the optimiser, so that addresses derived from it after this
+// point are recomputed rather than kept live (in VGPRs) across the code in between.
+template <class T> __device__ inline void opaque(T &x) { asm volatile("" : "+v"(x)); }
+
+template <int NS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) finish_species_kernel(Box b, double *__restrict__ w,
+                                                             const double *__restrict__ wa, double *__restrict__ fl,
+                                                             double *__restrict__ fln, FinishArgs F) {
+  using X = Idx<NS>;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  long pt;
+  if (!cell_of<NS>(b, t, i, j, k, pt)) return;
+  const long ps = b.pstride(), fs_ = b.fstride();
+  const double gamdt = F.gamdt;
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = fl[t + fs_ * s];
+  if (F.chem) {
+    const double rho = wa[pt + ps * (X::J_R - 1)];
+    const double tt = wa[pt + ps * (X::J_T - 1)];
+    double conc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) conc[s] = rho * wa[pt + ps * s] / F.mw[s];
+    chemistry_source_packed<NS>(F.K, tt, conc, acc);
+  }
+  opaque(t);
+  opaque(pt);
+  const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    fl[t + fs_ * s] = acc[s];
+    const double fn = fln[t + fs_ * s] - gamdt * acc[s];
+    fln[t + fs_ * s] = fn;
+    if (fluid) w[pt + ps * s] = w[pt + ps * s] + fn;
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(256) finish_mom_kernel(Box b, double *__restrict__ w, const double *__restrict__ wa,
+                                                         double *__restrict__ fl, double *__restrict__ fln,
+                                                         FinishArgs F) {
   using X = Idx<NS>;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.fstride()) return;
-  const int i = 1 + (int)(t % b.nx);
-  const long r = t / b.nx;
-  const int j = 1 + (int)(r % b.ny);
-  const int k = 1 + (int)(r / b.ny);
-  const long pt = b.p(i, j, k), ps = b.pstride(), fs_ = b.fstride();
+  int i, j, k;
+  long pt;
+  if (!cell_of<NS>(b, t, i, j, k, pt)) return;
+  const long ps = b.pstride(), fs_ = b.fstride();
   const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
   const double gamdt = F.gamdt;
-  // ---- species: explicit chemistry (compute_chemistry_cuf) and the RK update
-  {
-    double acc[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) acc[s] = fl[t + fs_ * s];
-    if (F.chem) {
-      const double rho = wa[pt + ps * (X::J_R - 1)];
-      const double tt = wa[pt + ps * (X::J_T - 1)];
-      double conc[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) conc[s] = rho * wa[pt + ps * s] / F.mw[s];
-      chemistry_source_packed<NS>(F.K, tt, conc, acc);
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      fl[t + fs_ * s] = acc[s];
-      const double fn = fln[t + fs_ * s] - gamdt * acc[s];
-      fln[t + fs_ * s] = fn;
-      if (fluid) w[pt + ps * s] = w[pt + ps * s] + fn;
-    }
-  }
-  // ---- momentum and energy: reduced viscous terms (visflx_reduced_cuf) and the RK update
   double acc[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) acc[m] = fl[t + fs_ * (NS + m)];
@@ -189,6 +215,7 @@ extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int n
   FinishArgs F{coeff_deriv1, dcsidx, detady, dzitdz, fluid_mask, mw,
                KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                lmax, reduced, chem && nreactions > 0, gamdt};
-  STREAMS_DISPATCH_NS(n_s, (finish_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F)));
+  STREAMS_DISPATCH_NS(n_s, (finish_species_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F),
+                            finish_mom_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F)));
   return hip_status();
 }
