@@ -17,22 +17,18 @@ struct AuxTables {
   const int *nainsp;
 };
 
+// The thermodynamic state (eval_aux_cuf up to the mixture fraction, kernels_gpu.F90:6154-6251)
+// and the mixture transport (:6252-6300) run as two kernels over the same points: apart,
+// each holds far fewer registers than the pair would, for the price of re-reading Y, T, p
+// and rho (12 of the ~45 doubles a point moves).
 template <int NS, int LO>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) eval_aux_kernel(Box b, int istart, int jstart, int kstart,
+__global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int jstart, int kstart,
                                                        int ni, int nj, long npts, double *__restrict__ w,
                                                        double *__restrict__ w_aux, Thermo th,
                                                        AuxTables tb, double tol_iter_nr) {
   using X = Idx<NS>;
-  __shared__ double c_inv[NS * NS];  // 1/sqrt(1 + W_l/W_m)
-  __shared__ double c_pow[NS * NS];  // (W_m/W_l)^(1/4)
   __shared__ double tlds[thermo_lds_doubles<NS>(TH_CP | TH_CV | TH_CVH)];
   const ThermoLds TL = thermo_stage<NS, LO>(th, tlds, TH_CP | TH_CV | TH_CVH);
-  for (int q = threadIdx.x; q < NS * NS; q += blockDim.x) {
-    int l = q % NS, m = q / NS;
-    c_inv[q] = 1.0 / sqrt(1.0 + tb.mw[l] * tb.mwinv[m]);
-    c_pow[q] = pow(tb.mw[m] * tb.mwinv[l], 0.25);
-  }
-  __syncthreads();
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npts) return;
   int i = istart + (int)(t % ni);
@@ -97,6 +93,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) e
     zz = (beta - tb.beta0[1]) / (tb.beta0[0] - tb.beta0[1]);
   }
   w_aux[pt + ps * (X::J_Z - 1)] = zz;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, int jstart, int kstart, int ni,
+                                                             int nj, long npts, double *__restrict__ w_aux,
+                                                             AuxTables tb) {
+  using X = Idx<NS>;
+  __shared__ double c_inv[NS * NS];  // 1/sqrt(1 + W_l/W_m)
+  __shared__ double c_pow[NS * NS];  // (W_m/W_l)^(1/4)
+  for (int q = threadIdx.x; q < NS * NS; q += blockDim.x) {
+    int l = q % NS, m = q / NS;
+    c_inv[q] = 1.0 / sqrt(1.0 + tb.mw[l] * tb.mwinv[m]);
+    c_pow[q] = pow(tb.mw[m] * tb.mwinv[l], 0.25);
+  }
+  __syncthreads();
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  int i = istart + (int)(t % ni);
+  long r = t / ni;
+  int j = jstart + (int)(r % nj);
+  int k = kstart + (int)(r / nj);
+  const long pt = b.p(i, j, k), ps = b.pstride();
+  double yy[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) yy[s] = w_aux[pt + ps * s];
+  const double tt = w_aux[pt + ps * (X::J_T - 1)];
+  const double rho = w_aux[pt + ps * (X::J_R - 1)];
+  const double pp = w_aux[pt + ps * (X::J_P - 1)];
+  double rmix = tb.rgas[0] * yy[0];
+#pragma unroll
+  for (int s = 1; s < NS; ++s) rmix = rmix + tb.rgas[s] * yy[s];
   // table row
   int itt = (int)((tt - tb.t_min_tab) / tb.dt_tab) + 1;
   itt = itt < 1 ? 1 : itt;
@@ -106,16 +133,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) e
   const long nt = tb.num_t_tab + 1;
   // Mathur conductivity
   const double mwmixt = tb.r_univ / rmix;
-  double xs[NS];
   double k1 = 0.0, k2 = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const double *lp = tb.lambda + (itt - 1) + nt * s;
     double l0 = lp[0], l1 = lp[1];
     double klsp = l0 + (l1 - l0) * dtt;
-    xs[s] = yy[s] * mwmixt * tb.mwinv[s];
-    k1 = k1 + xs[s] * klsp;
-    k2 = k2 + xs[s] / klsp;
+    const double xs = yy[s] * mwmixt * tb.mwinv[s];
+    k1 = k1 + xs * klsp;
+    k2 = k2 + xs / klsp;
   }
   const double k_cond = 0.5 * (k1 + 1.0 / k2);
   // Wilke viscosity: sqrt(mu_m/mu_l) evaluated as sqrt(mu_m) * (1/sqrt(mu_l))
@@ -127,6 +153,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) e
     mus[s] = v0 + (v1 - v0) * dtt;
     smu[s] = sqrt(mus[s]);
   }
+  __builtin_amdgcn_sched_barrier(0);
   double mu = 0.0;
 #pragma unroll 1
   for (int l = 0; l < NS; ++l) {
@@ -143,7 +170,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) e
   mu = sqrt(8.0) * mu;
   w_aux[pt + ps * (X::J_MU - 1)] = mu;
   w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond;
-  // Bird species diffusion into the mixture
+  // Bird species diffusion into the mixture (mole fractions rebuilt here rather than kept
+  // live through the Wilke sums)
+  double mwx = mwmixt;
+  opaque(mwx);
+  double xs[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) xs[s] = yy[s] * mwx * tb.mwinv[s];
   const double fac = tb.p0 / pp * rho;
 #pragma unroll 1
   for (int l = 0; l < NS; ++l) {
@@ -234,7 +267,9 @@ extern "C" int streams_eval_aux(int n_s, int nx, int ny, int nz, int ng, int nv,
   long npts = (long)ni * nj * nk;
   STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r,
                                (eval_aux_kernel<NS, LO><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
-                                   b, istart, jstart, kstart, ni, nj, npts, w, w_aux, th, tb, tol_iter_nr))));
+                                   b, istart, jstart, kstart, ni, nj, npts, w, w_aux, th, tb, tol_iter_nr),
+                                eval_transport_kernel<NS><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
+                                   b, istart, jstart, kstart, ni, nj, npts, w_aux, tb))));
   return hip_status();
 }
 

@@ -31,6 +31,10 @@ struct Box {
   __host__ __device__ inline long fstride() const { return (long)nx * ny * nz; }
 };
 
+// Hides the value of x from the optimiser, so that values derived from it after this point
+// are recomputed rather than kept live (in VGPRs) across the code in between.
+template <class T> __device__ inline void opaque(T &x) { asm volatile("" : "+v"(x)); }
+
 // index of variables (index_define.h), 1-based as in the reference
 template <int NS> struct Idx {
   static constexpr int I_U = NS + 1, I_V = NS + 2, I_W = NS + 3, I_E = NS + 4;

@@ -36,10 +36,6 @@ __device__ inline bool cell_of(const Box &b, long t, int &i, int &j, int &k, lon
   return true;
 }
 
-// Hides the value of x from the optimiser, so that addresses derived from it after this
-// point are recomputed rather than kept live (in VGPRs) across the code in between.
-template <class T> __device__ inline void opaque(T &x) { asm volatile("" : "+v"(x)); }
-
 template <int NS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) finish_species_kernel(Box b, double *__restrict__ w,
                                                              const double *__restrict__ wa, double *__restrict__ fl,

@@ -126,14 +126,19 @@ __device__ inline double species_h(const double *rows, int s, int nr, const TPow
 template <int NS, int LO>
 __device__ inline double mixture_cp_lds(const ThermoLds &t, const int *nrange, const double *yy, const TPow<LO> &P) {
   using N = Nasa<LO>;
+  double cp_l[N::NL];
+#pragma unroll
+  for (int q = 0; q < N::NL; ++q) cp_l[q] = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {          // species by species, fenced as in the Newton rebuild
+    const int row = ((nrange[s] - 1) * NS + s) * TSTRIDE;
+#pragma unroll
+    for (int q = 0; q < N::NL; ++q) cp_l[q] = cp_l[q] + t.cp[row + q] * yy[s];
+    __builtin_amdgcn_sched_barrier(0);
+  }
   double cploc = 0.0;
 #pragma unroll
-  for (int q = 0; q < N::NL; ++q) {
-    double cp_l = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) cp_l = cp_l + t.cp[((nrange[s] - 1) * NS + s) * TSTRIDE + q] * yy[s];
-    cploc = cploc + cp_l * P[LO + q];
-  }
+  for (int q = 0; q < N::NL; ++q) cploc = cploc + cp_l[q] * P[LO + q];
   return cploc;
 }
 
@@ -181,20 +186,22 @@ __device__ inline double temperature_newton_lds(double ee, double t_start, const
       }
     }
     if (dirty) {
+      // species by species (the same sums in the same order as per coefficient), with a
+      // scheduling fence after each: left free, the scheduler issues all 11*NS LDS reads
+      // at once and the kernel's register count doubles
       sumb = 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) sumb = sumb + c_rows[((nrange[s] - 1) * NS + s) * TSTRIDE + N::NL] * yy[s];
+      for (int q = 0; q < N::NL; ++q) cvl[q] = cvh[q] = 0.0;
 #pragma unroll
-      for (int q = 0; q < N::NL; ++q) {
-        double a = 0.0, h = 0.0;
+      for (int s = 0; s < NS; ++s) {
+        const int row = ((nrange[s] - 1) * NS + s) * TSTRIDE;
+        sumb = sumb + c_rows[row + N::NL] * yy[s];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          const int row = ((nrange[s] - 1) * NS + s) * TSTRIDE + q;
-          a = a + c_rows[row] * yy[s];
-          h = h + ch_rows[row] * yy[s];
+        for (int q = 0; q < N::NL; ++q) {
+          cvl[q] = cvl[q] + c_rows[row + q] * yy[s];
+          cvh[q] = cvh[q] + ch_rows[row + q] * yy[s];
         }
-        cvl[q] = a;
-        cvh[q] = h;
+        __builtin_amdgcn_sched_barrier(0);
       }
       dirty = false;
     }
