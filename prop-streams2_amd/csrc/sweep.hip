@@ -31,7 +31,6 @@ namespace streams {
 template <int DIR> struct TileCfg { static constexpr int C = 16, S1 = 16; };
 template <> struct TileCfg<1> { static constexpr int C = 8, S1 = 32; };
 constexpr int LMX = 4;                      // largest half stencil (ep_order 8)
-constexpr int NPF = LMX * (LMX + 1) / 2;    // pair factors for lmax <= LMX
 constexpr int KCH = 6;                      // LDS slots per staging round
 
 struct SweepArgs {
@@ -56,17 +55,20 @@ struct Tile {
 
 // Pair factors and the momentum / energy fluxes of face f from the round-1 slots
 // (0 rho, 1 u, 2 v, 3 w, 4 h, 5 p); rows are cell positions minus p0.
-template <int DIR>
+template <int DIR, int LB>
 __device__ inline void keep_round1(const double *sb, int c, int f, int p0, int lmax, int nkeep, const double *cd1,
                                    bool zero, double *pf, double *fme) {
   using T = Tile<DIR>;
   auto V = [&](int slot, int p) { return sb[slot * T::SLOT + T::at(c, p - p0)]; };
   constexpr int SN = DIR == 1 ? 1 : DIR == 2 ? 2 : 3;   // slot of the normal velocity
   double ft2 = 0, ft3 = 0, ft4 = 0, ft5 = 0, ft6 = 0;
-  int q = 0;
-  for (int l = 1; l <= lmax; ++l) {
+#pragma unroll
+  for (int l = 1; l <= LB; ++l) {
+    if (l > lmax) break;
     double uvs2 = 0, uvs3 = 0, uvs4 = 0, uvs5 = 0, uvs5_i = 0, uvs5_k = 0, uvs5_p = 0, uvs6 = 0;
-    for (int m = 0; m <= l - 1; ++m, ++q) {
+#pragma unroll
+    for (int m = 0; m <= l - 1; ++m) {
+      const int q = l * (l - 1) / 2 + m;
       const int a = f - m, b = f - m + l;
       const double rhoi = V(0, a), uui = V(1, a), vvi = V(2, a), wwi = V(3, a), enti = V(4, a), ppi = V(5, a);
       const double rhoip = V(0, b), uuip = V(1, b), vvip = V(2, b), wwip = V(3, b), entip = V(4, b), ppip = V(5, b);
@@ -115,6 +117,7 @@ __device__ inline void keep_round1(const double *sb, int c, int f, int p0, int l
       }
     }
     const double cl = cd1[(l - 1) + 4 * (lmax - 1)];
+    __builtin_amdgcn_sched_barrier(0);     // one stencil width at a time in registers
     ft2 = ft2 + cl * uvs2;
     ft3 = ft3 + cl * uvs3;
     ft4 = ft4 + cl * uvs4;
@@ -129,22 +132,24 @@ __device__ inline void keep_round1(const double *sb, int c, int f, int p0, int l
 }
 
 // species flux of face f from one staged mass-fraction slot
-template <int DIR>
+template <int DIR, int LB>
 __device__ inline double keep_species(const double *slot, int c, int f, int p0, int lmax, const double *cd1,
                                       const double *pf, bool zero) {
   using T = Tile<DIR>;
   double ft1 = 0.0;
-  int q = 0;
-  for (int l = 1; l <= lmax; ++l) {
+#pragma unroll
+  for (int l = 1; l <= LB; ++l) {
+    if (l > lmax) break;
     double uvs1 = 0.0;
-    for (int m = 0; m <= l - 1; ++m, ++q)
-      uvs1 = uvs1 + pf[q] * (slot[T::at(c, f - m - p0)] + slot[T::at(c, f - m + l - p0)]);
+#pragma unroll
+    for (int m = 0; m <= l - 1; ++m)
+      uvs1 = uvs1 + pf[l * (l - 1) / 2 + m] * (slot[T::at(c, f - m - p0)] + slot[T::at(c, f - m + l - p0)]);
     ft1 = ft1 + cd1[(l - 1) + 4 * (lmax - 1)] * uvs1;
   }
   return zero ? 0.0 : 0.25 * ft1;
 }
 
-template <int NS, int DIR, int LO>
+template <int NS, int DIR, int LO, int LB>
 __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restrict__ wa, double *__restrict__ fl,
                                                     double *__restrict__ fln, const double *__restrict__ fhat,
                                                     SweepArgs A) {
@@ -214,7 +219,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
   }
   const double met = upd ? E.metric[f - 1] : 0.0;
   double acc[NV];
-  double pf[NPF];
+  double pf[LB * (LB + 1) / 2];
   // ---- round 1: rho, u, v, w, h, p
   {
     const int vars[6] = {X::J_R, X::J_U, X::J_V, X::J_W, X::J_H, X::J_P};
@@ -226,7 +231,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
 #pragma unroll
         for (int k = 0; k < 4; ++k) fme[k] = fhat[fidx + ps * (NS + k)];
       } else {
-        keep_round1<DIR>(sb, c, f, p0, lmax, E.nkeep, E.cd1, zero, pf, fme);
+        keep_round1<DIR, LB>(sb, c, f, p0, lmax, E.nkeep, E.cd1, zero, pf, fme);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) xb[k * T::XSLOT + me] = fme[k];
@@ -251,7 +256,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
 #pragma unroll 1
       for (int k = 0; k < nk; ++k)
         xb[k * T::XSLOT + me] = shock ? fhat[fidx + ps * (s0k + k)]
-                                      : keep_species<DIR>(sb + k * T::SLOT, c, f, p0, lmax, E.cd1, pf, zero);
+                                      : keep_species<DIR, LB>(sb + k * T::SLOT, c, f, p0, lmax, E.cd1, pf, zero);
     }
     __syncthreads();
     if (upd) {
@@ -265,14 +270,20 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
     const ViscArgs &V = A.V;
     const int a = f, cc = f + 1;
     double sigx = 0, sigy = 0, sigz = 0, sigq = 0, mwm = 1, mwmp = 1, mwmif = 0, fsum = 0, ta = 0, tc = 0;
+    // the two cells' offsets in w_aux; opaque() before each phase makes the compiler
+    // rebuild the per-variable addresses (one add per load) instead of keeping ~20 of them
+    // live, and the fences bound the loads in flight
+    long ga = G.base + (long)a * G.st, gc = ga + G.st;
+    auto CA = [&](int m) { return wa[ga + ps * (m - 1)]; };
+    auto CC = [&](int m) { return wa[gc + ps * (m - 1)]; };
     if (valid) {
-      const double uu = G(a, X::J_U), uup = G(cc, X::J_U);
-      const double vv = G(a, X::J_V), vvp = G(cc, X::J_V);
-      const double ww = G(a, X::J_W), wwp = G(cc, X::J_W);
-      ta = G(a, X::J_T);
-      tc = G(cc, X::J_T);
-      const double mu = G(a, X::J_MU), mup = G(cc, X::J_MU);
-      const double kc = G(a, X::J_K_COND), kcp = G(cc, X::J_K_COND);
+      const double uu = CA(X::J_U), uup = CC(X::J_U);
+      const double vv = CA(X::J_V), vvp = CC(X::J_V);
+      const double ww = CA(X::J_W), wwp = CC(X::J_W);
+      ta = CA(X::J_T);
+      tc = CC(X::J_T);
+      const double mu = CA(X::J_MU), mup = CC(X::J_MU);
+      const double kc = CA(X::J_K_COND), kcp = CC(X::J_K_COND);
       const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
       const double qqp = 0.5 * (uup * uup + vvp * vvp + wwp * wwp);
       const double dx = V.coord[f + 1 + b.ng - 1] - V.coord[f + b.ng - 1];
@@ -284,20 +295,28 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
       sigy = (vvp - vv) * muf;
       sigz = (wwp - ww) * muf;
       sigq = (tc - ta) * kcf + (qqp - qq) * muf;
-      double rm = V.rgas[0] * G(a, 1), rmp = V.rgas[0] * G(cc, 1);
+      __builtin_amdgcn_sched_barrier(0);
+      opaque(ga);
+      opaque(gc);
+      double rm = V.rgas[0] * CA(1), rmp = V.rgas[0] * CC(1);
 #pragma unroll
       for (int s = 1; s < NS; ++s) {
-        rm = rm + V.rgas[s] * G(a, s + 1);
-        rmp = rmp + V.rgas[s] * G(cc, s + 1);
+        rm = rm + V.rgas[s] * CA(s + 1);
+        rmp = rmp + V.rgas[s] * CC(s + 1);
+        if (s % 3 == 0) __builtin_amdgcn_sched_barrier(0);
       }
+      __builtin_amdgcn_sched_barrier(0);
       mwm = V.r_univ / rm;
       mwmp = V.r_univ / rmp;
       mwmif = 0.5 * (mwmp + mwm) / (mwmp * mwm);
       mwmif = mwmif / dx;
+      opaque(ga);
+      opaque(gc);
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        const double rd = 0.5 * (G(a, X::J_D_START + s + 1) + G(cc, X::J_D_START + s + 1));
-        fsum = fsum + (G(cc, s + 1) * mwmp - G(a, s + 1) * mwm) * rd;
+        const double rd = 0.5 * (CA(X::J_D_START + s + 1) + CC(X::J_D_START + s + 1));
+        fsum = fsum + (CC(s + 1) * mwmp - CA(s + 1) * mwm) * rd;
+        if (s % 3 == 2) __builtin_amdgcn_sched_barrier(0);
       }
       fsum = fsum * mwmif;
     }
@@ -308,11 +327,13 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
       const int nk = NS - s0k < KCH ? NS - s0k : KCH;
       __syncthreads();                      // xb free
       if (valid) {
+        opaque(ga);
+        opaque(gc);
 #pragma unroll 1
         for (int k = 0; k < nk; ++k) {
           const int s = s0k + k;
-          const double ya = G(a, s + 1), yc = G(cc, s + 1);
-          const double rd = 0.5 * (G(a, X::J_D_START + s + 1) + G(cc, X::J_D_START + s + 1));
+          const double ya = CA(s + 1), yc = CC(s + 1);
+          const double rd = 0.5 * (CA(X::J_D_START + s + 1) + CC(X::J_D_START + s + 1));
           const double sig = -rd * (yc * mwmp - ya * mwm) * mwmif + fsum * (0.5 * (yc + ya));
           const double hh = species_h<NS, LO>(TL.cph, s, range_of<NS>(TL, s, ta), TPow<LO>(ta));
           const double hhp = species_h<NS, LO>(TL.cph, s, range_of<NS>(TL, s, tc), TPow<LO>(tc));
@@ -373,7 +394,12 @@ int sweep_launch(const Box &b, SweepArgs A, hipStream_t st) {
   A.ncb = ncb;
   A.nseg = nseg;
   A.ntiles = (unsigned)ntiles;
-  sweep_kernel<NS, DIR, LO><<<xcd_grid(A.ntiles), 256, 0, st>>>(b, A.E.w_aux, A.E.fl, A.fln, A.E.fhat, A);
+  // the KEEP pair loops are unrolled up to a compile-time bound on lmax (ep_ord_change only
+  // lowers the order): 3 for the sixth-order scheme, LMX for the others
+  if (A.E.lmax_base == 3)
+    sweep_kernel<NS, DIR, LO, 3><<<xcd_grid(A.ntiles), 256, 0, st>>>(b, A.E.w_aux, A.E.fl, A.fln, A.E.fhat, A);
+  else
+    sweep_kernel<NS, DIR, LO, LMX><<<xcd_grid(A.ntiles), 256, 0, st>>>(b, A.E.w_aux, A.E.fl, A.fln, A.E.fhat, A);
   return hip_status();
 }
 
