@@ -155,8 +155,8 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
   }
   __builtin_amdgcn_sched_barrier(0);
   double mu = 0.0;
-#pragma unroll 1
-  for (int l = 0; l < NS; ++l) {
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {      // unrolled (no register indexing), one l at a time
     const double isl = 1.0 / smu[l];
     double den = 0.0;
 #pragma unroll
@@ -166,12 +166,15 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
       den = den + yy[m] * phi * tb.mwinv[m];
     }
     mu = mu + mus[l] * yy[l] * tb.mwinv[l] / den;
+    __builtin_amdgcn_sched_barrier(0);
   }
   mu = sqrt(8.0) * mu;
   w_aux[pt + ps * (X::J_MU - 1)] = mu;
   w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond;
   // Bird species diffusion into the mixture (mole fractions rebuilt here rather than kept
-  // live through the Wilke sums)
+  // live through the Wilke sums; the fence keeps the table reads below from being hoisted
+  // into the Wilke loop)
+  __builtin_amdgcn_sched_barrier(0);
   double mwx = mwmixt;
   opaque(mwx);
   double xs[NS];

@@ -70,8 +70,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) f
   }
 }
 
-template <int NS>
-__global__ void __launch_bounds__(256) finish_mom_kernel(Box b, double *__restrict__ w, const double *__restrict__ wa,
+template <int NS, int LB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) finish_mom_kernel(Box b, double *__restrict__ w, const double *__restrict__ wa,
                                                          double *__restrict__ fl, double *__restrict__ fln,
                                                          FinishArgs F) {
   using X = Idx<NS>;
@@ -82,9 +82,7 @@ __global__ void __launch_bounds__(256) finish_mom_kernel(Box b, double *__restri
   const long ps = b.pstride(), fs_ = b.fstride();
   const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
   const double gamdt = F.gamdt;
-  double acc[4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) acc[m] = fl[t + fs_ * (NS + m)];
+  double red[4] = {0.0, 0.0, 0.0, 0.0};   // reduced viscous terms, applied after the stencil
   if (F.reduced) {
     const long sx = 1, sy = b.px(), sz = b.px() * b.py();
     const double *U = wa + ps * (X::J_U - 1), *V = wa + ps * (X::J_V - 1), *Wv = wa + ps * (X::J_W - 1);
@@ -93,23 +91,42 @@ __global__ void __launch_bounds__(256) finish_mom_kernel(Box b, double *__restri
     double ux = 0, vx = 0, wx = 0, mux = 0, dvx = 0, uy = 0, vy = 0, wy = 0, muy = 0, dvy = 0;
     double uz = 0, vz = 0, wz = 0, muz = 0, dvz = 0;
     const int lmax = F.lmax;
-    for (int l = 1; l <= lmax; ++l) {
+    // unrolled to the compile-time bound LB >= lmax, one stencil offset of one direction
+    // (10 loads) in flight at a time: the kernel then fits 6 waves per SIMD
+#pragma unroll
+    for (int l = 1; l <= LB; ++l) {
+      if (l > lmax) break;
       const double ccl = F.cd1[(l - 1) + 4 * (lmax - 1)];
-      ux = ux + ccl * (U[pt + l * sx] - U[pt - l * sx]);
-      vx = vx + ccl * (V[pt + l * sx] - V[pt - l * sx]);
-      wx = wx + ccl * (Wv[pt + l * sx] - Wv[pt - l * sx]);
-      mux = mux + ccl * (MU[pt + l * sx] - MU[pt - l * sx]);
-      dvx = dvx + ccl * (DV[pt + l * sx] - DV[pt - l * sx]);
-      uy = uy + ccl * (U[pt + l * sy] - U[pt - l * sy]);
-      vy = vy + ccl * (V[pt + l * sy] - V[pt - l * sy]);
-      wy = wy + ccl * (Wv[pt + l * sy] - Wv[pt - l * sy]);
-      muy = muy + ccl * (MU[pt + l * sy] - MU[pt - l * sy]);
-      dvy = dvy + ccl * (DV[pt + l * sy] - DV[pt - l * sy]);
-      uz = uz + ccl * (U[pt + l * sz] - U[pt - l * sz]);
-      vz = vz + ccl * (V[pt + l * sz] - V[pt - l * sz]);
-      wz = wz + ccl * (Wv[pt + l * sz] - Wv[pt - l * sz]);
-      muz = muz + ccl * (MU[pt + l * sz] - MU[pt - l * sz]);
-      dvz = dvz + ccl * (DV[pt + l * sz] - DV[pt - l * sz]);
+      // a fresh copy of the point index per group makes the compiler rebuild the ten
+      // addresses from it rather than keep dozens of 64-bit addresses live
+      long q = pt;
+      opaque(q);
+      ux = ux + ccl * (U[q + l * sx] - U[q - l * sx]);
+      vx = vx + ccl * (V[q + l * sx] - V[q - l * sx]);
+      wx = wx + ccl * (Wv[q + l * sx] - Wv[q - l * sx]);
+      mux = mux + ccl * (MU[q + l * sx] - MU[q - l * sx]);
+      dvx = dvx + ccl * (DV[q + l * sx] - DV[q - l * sx]);
+      __builtin_amdgcn_sched_barrier(0);
+      q = pt + l * sy;
+      opaque(q);
+      long qm = pt - l * sy;
+      opaque(qm);
+      uy = uy + ccl * (U[q] - U[qm]);
+      vy = vy + ccl * (V[q] - V[qm]);
+      wy = wy + ccl * (Wv[q] - Wv[qm]);
+      muy = muy + ccl * (MU[q] - MU[qm]);
+      dvy = dvy + ccl * (DV[q] - DV[qm]);
+      __builtin_amdgcn_sched_barrier(0);
+      q = pt + l * sz;
+      opaque(q);
+      qm = pt - l * sz;
+      opaque(qm);
+      uz = uz + ccl * (U[q] - U[qm]);
+      vz = vz + ccl * (V[q] - V[qm]);
+      wz = wz + ccl * (Wv[q] - Wv[qm]);
+      muz = muz + ccl * (MU[q] - MU[qm]);
+      dvz = dvz + ccl * (DV[q] - DV[qm]);
+      __builtin_amdgcn_sched_barrier(0);
     }
     const double dx = F.dcsidx[i - 1], dy = F.detady[j - 1], dz = F.dzitdz[k - 1];
     ux *= dx; vx *= dx; wx *= dx; mux *= dx; dvx *= dx;
@@ -124,16 +141,17 @@ __global__ void __launch_bounds__(256) finish_mom_kernel(Box b, double *__restri
     const double sigah = (s11 * ux + s12 * uy + s13 * uz + s21 * vx + s22 * vy + s23 * vz + s31 * wx + s32 * wy +
                           s33 * wz) * mu;
     const double sigq = sigx * uu + sigy * vv + sigz * ww + sigah;
-    acc[0] = acc[0] - sigx;
-    acc[1] = acc[1] - sigy;
-    acc[2] = acc[2] - sigz;
-    acc[3] = acc[3] - sigq;
+    red[0] = sigx;
+    red[1] = sigy;
+    red[2] = sigz;
+    red[3] = sigq;
   }
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const long q = t + fs_ * (NS + m);
-    fl[q] = acc[m];
-    const double fn = fln[q] - gamdt * acc[m];
+    const double acc = F.reduced ? fl[q] - red[m] : fl[q];
+    fl[q] = acc;
+    const double fn = fln[q] - gamdt * acc;
     fln[q] = fn;
     if (fluid) w[pt + ps * (NS + m)] = w[pt + ps * (NS + m)] + fn;
   }
@@ -212,6 +230,7 @@ extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int n
                KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                lmax, reduced, chem && nreactions > 0, gamdt};
   STREAMS_DISPATCH_NS(n_s, (finish_species_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F),
-                            finish_mom_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F)));
+                            (lmax == 3 ? finish_mom_kernel<NS, 3><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F)
+                                       : finish_mom_kernel<NS, 4><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F))));
   return hip_status();
 }
