@@ -197,36 +197,47 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
   }
 }
 
-template <int NS>
+template <int NS, int LB>
 __global__ void __launch_bounds__(256) eval_aux2_kernel(Box b, int lmax, double *__restrict__ w_aux,
                                                         const double *__restrict__ cd1,
                                                         const double *__restrict__ dcsidx,
                                                         const double *__restrict__ detady,
                                                         const double *__restrict__ dzitdz,
-                                                        double eps_sensor, int sensor_type) {
+                                                        double eps_sensor, int sensor_type, StencilRows SR) {
   using X = Idx<NS>;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long n = (long)b.nx * b.ny * b.nz;
-  if (t >= n) return;
-  int i = 1 + (int)(t % b.nx);
-  long r = t / b.nx;
-  int j = 1 + (int)(r % b.ny);
-  int k = 1 + (int)(r / b.ny);
+  int i, j, k;
+  if (!stencil_point(b, SR, i, j, k)) return;
   const long pt = b.p(i, j, k), ps = b.pstride();
   const long sx = 1, sy = b.px(), sz = b.px() * b.py();
   const double *U = w_aux + ps * (X::J_U - 1), *V = w_aux + ps * (X::J_V - 1), *Wv = w_aux + ps * (X::J_W - 1);
   double ux = 0, vx = 0, wx = 0, uy = 0, vy = 0, wy = 0, uz = 0, vz = 0, wz = 0;
-  for (int l = 1; l <= lmax; ++l) {
+  // one stencil offset of one direction in flight at a time (see finish_mom_kernel)
+#pragma unroll
+  for (int l = 1; l <= LB; ++l) {
+    if (l > lmax) break;
     double ccl = cd1[(l - 1) + 4 * (lmax - 1)];
-    ux = ux + ccl * (U[pt + l * sx] - U[pt - l * sx]);
-    vx = vx + ccl * (V[pt + l * sx] - V[pt - l * sx]);
-    wx = wx + ccl * (Wv[pt + l * sx] - Wv[pt - l * sx]);
-    uy = uy + ccl * (U[pt + l * sy] - U[pt - l * sy]);
-    vy = vy + ccl * (V[pt + l * sy] - V[pt - l * sy]);
-    wy = wy + ccl * (Wv[pt + l * sy] - Wv[pt - l * sy]);
-    uz = uz + ccl * (U[pt + l * sz] - U[pt - l * sz]);
-    vz = vz + ccl * (V[pt + l * sz] - V[pt - l * sz]);
-    wz = wz + ccl * (Wv[pt + l * sz] - Wv[pt - l * sz]);
+    long q = pt;
+    opaque(q);
+    ux = ux + ccl * (U[q + l * sx] - U[q - l * sx]);
+    vx = vx + ccl * (V[q + l * sx] - V[q - l * sx]);
+    wx = wx + ccl * (Wv[q + l * sx] - Wv[q - l * sx]);
+    __builtin_amdgcn_sched_barrier(0);
+    q = pt + l * sy;
+    opaque(q);
+    long qm = pt - l * sy;
+    opaque(qm);
+    uy = uy + ccl * (U[q] - U[qm]);
+    vy = vy + ccl * (V[q] - V[qm]);
+    wy = wy + ccl * (Wv[q] - Wv[qm]);
+    __builtin_amdgcn_sched_barrier(0);
+    q = pt + l * sz;
+    opaque(q);
+    qm = pt - l * sz;
+    opaque(qm);
+    uz = uz + ccl * (U[q] - U[qm]);
+    vz = vz + ccl * (V[q] - V[qm]);
+    wz = wz + ccl * (Wv[q] - Wv[qm]);
+    __builtin_amdgcn_sched_barrier(0);
   }
   const double dx = dcsidx[i - 1], dy = detady[j - 1], dz = dzitdz[k - 1];
   ux *= dx; vx *= dx; wx *= dx;
@@ -286,8 +297,14 @@ extern "C" int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv
   Box b{nx, ny, nz, ng};
   long n = (long)nx * ny * nz;
   if (n == 0) return 0;
-  STREAMS_DISPATCH_NS(n_s, (eval_aux2_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
-                                b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
-                                sensor_type)));
+  const StencilRows SR = stencil_rows(b);
+  if (SR.tiles(nz) > 0xffffff00u) return STREAMS_E_DIMS;
+  const unsigned gm = xcd_grid(SR.tiles(nz));
+  STREAMS_DISPATCH_NS(n_s, (lmax == 3 ? eval_aux2_kernel<NS, 3><<<gm, 256, 0, as_stream(stream)>>>(
+                                            b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                            sensor_type, SR)
+                                      : eval_aux2_kernel<NS, 4><<<gm, 256, 0, as_stream(stream)>>>(
+                                            b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                            sensor_type, SR)));
   return hip_status();
 }

@@ -217,6 +217,35 @@ __device__ inline unsigned xcd_tile(unsigned b, unsigned grid) {
   return (b % NXCD) * (grid / NXCD) + b / NXCD;   // a permutation of 0..grid-1
 }
 
+// Row order for pointwise kernels with a +-l stencil in y and z: one workgroup per
+// 256-point chunk of an i-row, the rows visited in bands of SB j-rows with k fastest within
+// a band, and the bands split into contiguous ranges per XCD.  The rows an XCD has in
+// flight then form a compact (j, k) block whose y and z neighbours sit in that XCD's L2.
+struct StencilRows {
+  static constexpr int SB = 8;                       // j-rows per band
+  int nib, nbands;                                   // i-chunks per row, bands
+  __host__ __device__ inline unsigned tiles(int nz) const { return (unsigned)nib * SB * nz * nbands; }
+};
+inline StencilRows stencil_rows(const Box &b) {
+  StencilRows r;
+  r.nib = (b.nx + 255) / 256;
+  r.nbands = (b.ny + StencilRows::SB - 1) / StencilRows::SB;
+  return r;
+}
+// Interior point (i, j, k) of this thread; false for padding tiles and points past nx / ny.
+__device__ inline bool stencil_point(const Box &b, const StencilRows &r, int &i, int &j, int &k) {
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= r.tiles(b.nz)) return false;
+  const unsigned jj = tile % StencilRows::SB;
+  const unsigned q = tile / StencilRows::SB;
+  const unsigned ib = q % r.nib;
+  const unsigned q2 = q / r.nib;
+  k = 1 + (int)(q2 % b.nz);
+  j = 1 + (int)((q2 / b.nz) * StencilRows::SB + jj);
+  i = 1 + (int)(ib * 256 + threadIdx.x);
+  return i <= b.nx && j <= b.ny;
+}
+
 // ------------------------------------------------------------------ launch helpers
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

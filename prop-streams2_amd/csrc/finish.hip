@@ -73,12 +73,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) f
 template <int NS, int LB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) finish_mom_kernel(Box b, double *__restrict__ w, const double *__restrict__ wa,
                                                          double *__restrict__ fl, double *__restrict__ fln,
-                                                         FinishArgs F) {
+                                                         FinishArgs F, StencilRows SR) {
   using X = Idx<NS>;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
-  long pt;
-  if (!cell_of<NS>(b, t, i, j, k, pt)) return;
+  if (!stencil_point(b, SR, i, j, k)) return;
+  const long t = b.f(i, j, k), pt = b.p(i, j, k);
   const long ps = b.pstride(), fs_ = b.fstride();
   const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
   const double gamdt = F.gamdt;
@@ -229,8 +228,11 @@ extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int n
   FinishArgs F{coeff_deriv1, dcsidx, detady, dzitdz, fluid_mask, mw,
                KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                lmax, reduced, chem && nreactions > 0, gamdt};
+  const StencilRows SR = stencil_rows(b);
+  if (SR.tiles(nz) > 0xffffff00u) return STREAMS_E_DIMS;
+  const unsigned gm = xcd_grid(SR.tiles(nz));
   STREAMS_DISPATCH_NS(n_s, (finish_species_kernel<NS><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F),
-                            (lmax == 3 ? finish_mom_kernel<NS, 3><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F)
-                                       : finish_mom_kernel<NS, 4><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F))));
+                            (lmax == 3 ? finish_mom_kernel<NS, 3><<<gm, 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F, SR)
+                                       : finish_mom_kernel<NS, 4><<<gm, 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F, SR))));
   return hip_status();
 }

@@ -54,7 +54,8 @@ struct Tile {
 };
 
 // Pair factors and the momentum / energy fluxes of face f from the round-1 slots
-// (0 rho, 1 u, 2 v, 3 w, 4 h, 5 p); rows are cell positions minus p0.
+// (0 rho, 1 u, 2 v, 3 w, 4 h -- or the internal energy e when nkeep >= 0 --, 5 p); rows are
+// cell positions minus p0.
 template <int DIR, int LB>
 __device__ inline void keep_round1(const double *sb, int c, int f, int p0, int lmax, int nkeep, const double *cd1,
                                    bool zero, double *pf, double *fme) {
@@ -87,8 +88,7 @@ __device__ inline void keep_round1(const double *sb, int c, int f, int p0, int l
         }
         const double uv_part = (uni + unip) * rhom * drhof;
         pf[q] = uv_part;
-        const double eei = enti - ppi / rhoi - 0.5 * (uui * uui + vvi * vvi + wwi * wwi);
-        const double eeip = entip - ppip / rhoip - 0.5 * (uuip * uuip + vvip * vvip + wwip * wwip);
+        const double eei = enti, eeip = entip;    // slot 4 holds e for nkeep >= 0
         const double eem = eei + eeip;
         double deef = 1.0;
         if (nkeep > 0) {
@@ -200,6 +200,32 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
       for (int k = 0; k < nvar; ++k) sb[k * T::SLOT + li] = wa[g + ps * (vars[k] - 1)];
     }
   };
+  // round 1 with the KEEP energy split: slot 4 holds the cell's internal energy
+  // e = h - p/rho - (u^2+v^2+w^2)/2 (the expression the pair loop would otherwise evaluate
+  // for every pair a cell takes part in) instead of h
+  auto stage_round1_e = [&]() {
+    for (int idx = t; idx < C * R; idx += 256) {
+      int cc, r;
+      if (DIR == 1) { cc = idx / R; r = idx % R; }
+      else { r = idx / C; cc = idx % C; }
+      const int p = p0 + r;
+      const int col = cbase + cc;
+      if (col > ncol || p < plo || p > phi) continue;
+      long base, st;
+      line_base<DIR>(b, col, t2, base, st);
+      const long g = base + (long)p * st;
+      const int li = T::at(cc, r);
+      const double rho = wa[g + ps * (X::J_R - 1)], uu = wa[g + ps * (X::J_U - 1)];
+      const double vv = wa[g + ps * (X::J_V - 1)], ww = wa[g + ps * (X::J_W - 1)];
+      const double ent = wa[g + ps * (X::J_H - 1)], pp = wa[g + ps * (X::J_P - 1)];
+      sb[0 * T::SLOT + li] = rho;
+      sb[1 * T::SLOT + li] = uu;
+      sb[2 * T::SLOT + li] = vv;
+      sb[3 * T::SLOT + li] = ww;
+      sb[4 * T::SLOT + li] = ent - pp / rho - 0.5 * (uu * uu + vv * vv + ww * ww);
+      sb[5 * T::SLOT + li] = pp;
+    }
+  };
   const int col = cbase + c;
   const int f = s0 - 1 + fl_;
   const bool valid = col <= ncol && f <= ue;
@@ -222,8 +248,12 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
   double pf[LB * (LB + 1) / 2];
   // ---- round 1: rho, u, v, w, h, p
   {
-    const int vars[6] = {X::J_R, X::J_U, X::J_V, X::J_W, X::J_H, X::J_P};
-    stage(vars, 6);
+    if (E.nkeep >= 0) {
+      stage_round1_e();
+    } else {
+      const int vars[6] = {X::J_R, X::J_U, X::J_V, X::J_W, X::J_H, X::J_P};
+      stage(vars, 6);
+    }
     __syncthreads();
     double fme[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
