@@ -77,6 +77,34 @@ class TimedBackend:
         return out
 
 
+# GPU kernels each timed entry point launches (the fused path, KEEP central, sixth order)
+ENTRY_KERNELS = {
+    "rhs_finish": ["finish_species_kernel<NS>", "finish_mom_kernel<NS, 3>"],
+    "rhs_sweep_1": ["sweep_kernel<NS, 1, 0, 3>"],
+    "rhs_sweep_2": ["sweep_kernel<NS, 2, 0, 3>"],
+    "rhs_sweep_3": ["sweep_kernel<NS, 3, 0, 3>"],
+    "eval_aux": ["eval_aux_kernel<NS, 0>", "eval_transport_kernel<NS>"],
+    "eval_aux2": ["eval_aux2_kernel<NS, 3>"],
+}
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256cube_h2air9.md"
+
+
+def measured_traffic(entry: str, n: int):
+    """HBM bytes per launch of `entry` from the FETCH_SIZE / WRITE_SIZE passes recorded in
+    the traffic summary (its ```traffic block); None off the 256^3 workload or when absent."""
+    if n != 256 or not TRAFFIC_FILE.exists():
+        return None
+    text = TRAFFIC_FILE.read_text()
+    if "```traffic" not in text:
+        return None
+    block = text.split("```traffic", 1)[1].split("```", 1)[0]
+    for line in block.strip().splitlines():
+        name, val = line.split()
+        if name == entry:
+            return int(val)
+    return None
+
+
 def cpu_baseline(pkg, tables, n_sample: int, steps: int):
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_backend
@@ -122,7 +150,8 @@ def copy_probe(gib: float):
 def reactor_bench(pkg, n: int, reps: int = 3):
     """Batched constant-volume reactors (T log-uniform 800-2500 K, p 0.5-20 atm, Dirichlet
     Y, horizon 1e-6 s): reactor-solves/s of the FP64 Rosenbrock batch and of its
-    mixed-precision variant, and the largest relative difference in T between the two."""
+    mixed-precision variant, and the relative differences in final T between the two (the
+    final state inverted for T on the host)."""
     import torch
     tables = pkg.cases.h2_air_mechanism().tables()
     w, tg, dttry, tend, info = pkg.reactors.reactor_inputs(tables, n)
@@ -159,6 +188,12 @@ def reactor_bench(pkg, n: int, reps: int = 3):
     scale = np.abs(b[:, fin]).max(axis=1)
     dif = np.abs(a[:, fin] - b[:, fin]).max(axis=1)
     out["mixed_vs_fp64_max_rel_diff_per_variable"] = float((dif[scale > 0] / scale[scale > 0]).max())
+    ta = pkg.reactors.final_temperatures(tables, a, tg)
+    tb_ = pkg.reactors.final_temperatures(tables, b, tg)
+    okT = np.isfinite(ta) & np.isfinite(tb_)
+    dT = np.abs(ta[okT] - tb_[okT]) / tb_[okT]
+    out["mixed_vs_fp64_final_T_rel_diff"] = {"p50": float(np.median(dT)), "p99": float(np.quantile(dT, 0.99)),
+                                             "max": float(dT.max()), "fraction_above_1e-6": float((dT > 1e-6).mean())}
     return out
 
 
@@ -187,7 +222,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=80)
+    ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--reactors", type=int, default=0,
                     help="also time the batched chemistry on this many reactors (e.g. 2097152)")
@@ -270,7 +305,9 @@ def main():
             "entry_ms_per_call": breakdown,
         },
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": rf.HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / rf.HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / rf.HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dominant, n),
+                     "gpu_kernels": [k.replace("NS", str(tables.n_species)) for k in ENTRY_KERNELS.get(dominant, [])],
                      "avg_launch_ms": dom_avg_s * 1e3, "algorithmic_bytes": dom_bytes},
     }
     if args.reactors > 0:

@@ -92,35 +92,40 @@ struct RosRhsPressure {
   }
 };
 
-// Mixed-precision variant of the same step (same ROS4 stages, step-size control and error
-// norm): the iteration matrix W = I/(gamma h) - J is factored in FP32 and every stage solve
-// is refined once against the FP64 residual b - W x, with J kept in FP64.  The Jacobian is
-// a one-sided difference (nv + 1 right-hand sides instead of 2 nv + 1): its O(h) error is
-// of the order of the FP32 factorisation's own rounding, which the FP64 refinement
-// removes from the stage solutions.  The right-hand sides, the temperature inversion and
-// the error estimate stay FP64.
-template <int NV>
+// Mixed-precision variant of the same method (same ROS4 stages, step-size control and error
+// norm), with three changes:
+//  * The linear algebra runs on the species block.  In a reactor the momentum and energy
+//    rows of the right-hand side are identically zero, so their Jacobian rows are zero, W's
+//    rows there are gi * e_m, and every stage solution has exactly zero in those components;
+//    the species components then solve W_ss g_s = b_s.  Solving that block alone gives the
+//    same stage vectors without the 4 extra columns.
+//  * The Jacobian is a one-sided difference over the N_S species columns (N_S + 1
+//    right-hand sides instead of 2 (N_S + 4) + 1).
+//  * W_ss is factored in FP32 and each stage solve is refined once against the FP64
+//    residual b - W x, with J kept in FP64; the right-hand sides, the temperature inversion
+//    and the error estimate stay FP64.
+template <int N>
 __device__ inline void solve_refined(const double *jac, const float *lu, const int *indx, double gi, double *b) {
-  float x32[NV];
+  float x32[N];
 #pragma unroll
-  for (int m = 0; m < NV; ++m) x32[m] = (float)b[m];
-  lubksb<NV, float>(lu, indx, x32);
-  double x[NV], r[NV];
+  for (int m = 0; m < N; ++m) x32[m] = (float)b[m];
+  lubksb<N, float>(lu, indx, x32);
+  double x[N], r[N];
 #pragma unroll
-  for (int m = 0; m < NV; ++m) x[m] = (double)x32[m];
+  for (int m = 0; m < N; ++m) x[m] = (double)x32[m];
   // r = b - W x with W = gi*I - J (FP64)
 #pragma unroll
-  for (int m = 0; m < NV; ++m) r[m] = b[m] - gi * x[m];
-  for (int n = 0; n < NV; ++n) {
+  for (int m = 0; m < N; ++m) r[m] = b[m] - gi * x[m];
+  for (int n = 0; n < N; ++n) {
     const double xn = x[n];
 #pragma unroll
-    for (int m = 0; m < NV; ++m) r[m] = r[m] + jac[m + NV * n] * xn;
+    for (int m = 0; m < N; ++m) r[m] = r[m] + jac[m + N * n] * xn;
   }
 #pragma unroll
-  for (int m = 0; m < NV; ++m) x32[m] = (float)r[m];
-  lubksb<NV, float>(lu, indx, x32);
+  for (int m = 0; m < N; ++m) x32[m] = (float)r[m];
+  lubksb<N, float>(lu, indx, x32);
 #pragma unroll
-  for (int m = 0; m < NV; ++m) b[m] = x[m] + (double)x32[m];
+  for (int m = 0; m < N; ++m) b[m] = x[m] + (double)x32[m];
 }
 
 template <int NS, class Rhs>
@@ -133,52 +138,52 @@ __device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, do
   const double c41 = -112.0 / 125.0, c42 = -54.0 / 125.0, c43 = -2.0 / 5.0;
   const double b1 = 19.0 / 9.0, b2 = 1.0 / 2.0, b3 = 25.0 / 108.0, b4 = 125.0 / 108.0;
   const double e1 = 17.0 / 54.0, e2 = 7.0 / 36.0, e3 = 0.0, e4 = 125.0 / 108.0;
-  double jac[NV * NV];
-  float lu[NV * NV];
-  double w_sav[NV], rhs_sav[NV], yscal[NV], rhs[NV], rp[NV];
-  double g1[NV], g2[NV], g3[NV], g4[NV];
-  int indx[NV];
+  double jac[NS * NS];
+  float lu[NS * NS];
+  double w_sav[NS], rhs_sav[NV], yscal[NS], rhs[NV], rp[NV];
+  double g1[NS], g2[NS], g3[NS], g4[NS];
+  int indx[NS];
   int finish = 0;
   double time = time_start, dt = dttry, itol = 1.0 / tol, dtnext = dt;
   for (int iter = 1; iter <= maxsteps; ++iter) {
     const double t0 = time;
     rhs_fn(wv, tg, rhs_sav);
-    for (int n = 0; n < NV; ++n) {
+    for (int n = 0; n < NS; ++n) {
       const double wtmp = wv[n];
       const double h = wtmp * 1.0e-06 + 1.0e-14;
       wv[n] = wtmp + h;
       double tj = tg;
       rhs_fn(wv, tj, rp);
       wv[n] = wtmp;
-      for (int m = 0; m < NV; ++m) jac[m + NV * n] = (rp[m] - rhs_sav[m]) / h;
+      for (int m = 0; m < NS; ++m) jac[m + NS * n] = (rp[m] - rhs_sav[m]) / h;
     }
-    for (int m = 0; m < NV; ++m) {
+    for (int m = 0; m < NS; ++m) {
       w_sav[m] = wv[m];
       yscal[m] = fmax(1.0, fabs(w_sav[m]));
     }
     for (int jtry = 1; jtry <= maxtry; ++jtry) {
       const double gi = 1.0 / (gam * dt);
-      for (int n = 0; n < NV; ++n) {
-        for (int m = 0; m < NV; ++m) lu[m + NV * n] = (float)(-jac[m + NV * n]);
-        lu[n + NV * n] = (float)(gi - jac[n + NV * n]);
+      for (int n = 0; n < NS; ++n) {
+        for (int m = 0; m < NS; ++m) lu[m + NS * n] = (float)(-jac[m + NS * n]);
+        lu[n + NS * n] = (float)(gi - jac[n + NS * n]);
       }
-      ludcmp<NV, float>(lu, indx);
-      for (int m = 0; m < NV; ++m) g1[m] = rhs_sav[m];
-      solve_refined<NV>(jac, lu, indx, gi, g1);
-      for (int m = 0; m < NV; ++m) wv[m] = w_sav[m] + a21 * g1[m];
+      ludcmp<NS, float>(lu, indx);
+      for (int m = 0; m < NS; ++m) g1[m] = rhs_sav[m];
+      solve_refined<NS>(jac, lu, indx, gi, g1);
+      for (int m = 0; m < NS; ++m) wv[m] = w_sav[m] + a21 * g1[m];
       time = t0 + dt;
       rhs_fn(wv, tg, rhs);
-      for (int m = 0; m < NV; ++m) g2[m] = rhs[m] + c21 * g1[m] / dt;
-      solve_refined<NV>(jac, lu, indx, gi, g2);
-      for (int m = 0; m < NV; ++m) wv[m] = w_sav[m] + a31 * g1[m] + a32 * g2[m];
+      for (int m = 0; m < NS; ++m) g2[m] = rhs[m] + c21 * g1[m] / dt;
+      solve_refined<NS>(jac, lu, indx, gi, g2);
+      for (int m = 0; m < NS; ++m) wv[m] = w_sav[m] + a31 * g1[m] + a32 * g2[m];
       time = t0 + 3.0 / 5.0 * dt;
       rhs_fn(wv, tg, rhs);
-      for (int m = 0; m < NV; ++m) g3[m] = rhs[m] + (c31 * g1[m] + c32 * g2[m]) / dt;
-      solve_refined<NV>(jac, lu, indx, gi, g3);
-      for (int m = 0; m < NV; ++m) g4[m] = rhs[m] + (c41 * g1[m] + c42 * g2[m] + c43 * g3[m]) / dt;
-      solve_refined<NV>(jac, lu, indx, gi, g4);
+      for (int m = 0; m < NS; ++m) g3[m] = rhs[m] + (c31 * g1[m] + c32 * g2[m]) / dt;
+      solve_refined<NS>(jac, lu, indx, gi, g3);
+      for (int m = 0; m < NS; ++m) g4[m] = rhs[m] + (c41 * g1[m] + c42 * g2[m] + c43 * g3[m]) / dt;
+      solve_refined<NS>(jac, lu, indx, gi, g4);
       double errmax = 0.0;
-      for (int m = 0; m < NV; ++m) {
+      for (int m = 0; m < NS; ++m) {
         wv[m] = w_sav[m] + b1 * g1[m] + b2 * g2[m] + b3 * g3[m] + b4 * g4[m];
         const double err = e1 * g1[m] + e2 * g2[m] + e3 * g3[m] + e4 * g4[m];
         errmax = fmax(errmax, fabs(err / yscal[m]));

@@ -66,3 +66,28 @@ def reactor_inputs(tables: Tables, n: int, seed: int = 7, dt: float = 1e-6, dttr
     from .cases import reactor_batch
     w, tg, info = reactor_batch(tables, n, seed=seed)
     return w, tg, np.full(n, dttry), np.full(n, dt), info
+
+
+def temperature_from_e(tables: Tables, e: np.ndarray, Y: np.ndarray, t_guess: np.ndarray) -> np.ndarray:
+    """Newton inversion of the mixture internal energy on the host for many reactor states
+    (the integrator's own temperature output belongs to its last right-hand side)."""
+    T = np.array(t_guess, dtype=np.float64)
+    r = Y @ tables.rgas
+    for _ in range(50):
+        f = tables.e_from_T_many(T, Y) - e
+        dT = f / (tables.cp_from_T_many(T, Y) - r)
+        T = T - dT
+        if np.all(np.abs(dT) <= 1e-11 * T):
+            break
+    return T
+
+
+def final_temperatures(tables: Tables, w: np.ndarray, t_guess: np.ndarray) -> np.ndarray:
+    """T of constant-volume reactor states w (N_S + 4, n), NaN where a state is not finite."""
+    ns = tables.n_species
+    ok = np.isfinite(w).all(axis=0)
+    out = np.full(w.shape[1], np.nan)
+    rho = w[:ns, ok].sum(axis=0)
+    Y = (w[:ns, ok] / rho).T
+    out[ok] = temperature_from_e(tables, w[ns + 3, ok] / rho, Y, t_guess[ok])
+    return out
