@@ -184,6 +184,28 @@ int streams_rosenbrock(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int 
                        double les_c_mix, double les_c_eps, const double *dcsidx,
                        const double *detady, const double *dzitdz, void *stream);
 
+/* rosenbrock_krylov_kernel, kernels_gpu.F90:10360-11205 (launched by rosenbrock_krylov,
+ * multideal_gpu.F90:769-838): per cell 4-stage Rosenbrock-Krylov (ROK) from time_start to
+ * time_end, a Krylov space of at most M_max = 4 vectors rebuilt by Arnoldi on one-sided
+ * finite-difference Jacobian-vector products for every attempted step, error norm
+ * ||(yt - ys) / (1e-4 ys + 1e-8)||_2, only for cells with 5e-4 < Z < 1 - 5e-4.
+ * simpler_splitting = 1 subtracts the transport forcing fl_sav(1:nx,1:ny,1:nz,1:nv) from
+ * every right-hand side.  w_aux is read only (T guess, Z).  enable_pasr must be 0. */
+int streams_rosenbrock_krylov(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
+                              double *w, double *w_aux, const double *fl, const double *mw,
+                              int nreactions, const double *arr_a, const double *arr_b,
+                              const double *arr_ea, const double *tb_eff,
+                              const double *falloff_coeffs, const int *reac_ty,
+                              const int *isrev, const double *r_coeffs, const double *p_coeffs,
+                              const double *kc_tab, int num_t_tab, double t_min_tab,
+                              double dt_tab, double r_univ, double tol_iter_nr,
+                              const double *cv_coeff, int nsetcv, const double *trange,
+                              int indx_cp_l, int indx_cp_r, double dttry, double time_start,
+                              double time_end, int simpler_splitting, const double *fl_sav,
+                              int maxsteps, int enable_pasr, double les_c_yoshi, double les_c_mix,
+                              double les_c_eps, const double *dcsidx, const double *detady,
+                              const double *dzitdz, void *stream);
+
 /* ---- Runge-Kutta bookkeeping and time step (a10, a12) ------------------------------ */
 
 /* init_flux_cuf (kernels_gpu.F90:28-45): fln = -rhodt*fl; fl = 0. */
@@ -265,9 +287,10 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
  * receive the accepted / rejected step counts.  Kinetics come from the device copy of
  * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
  * counter (zeroed by the call on the stream); wg_per_cu <= 0 sizes the persistent grid to
- * the resident capacity.  mixed = 0 is the FP64 integrator; mixed = 1 factors the iteration
- * matrix in FP32, refines every stage solve once against the FP64 residual and forms the
- * Jacobian by one-sided differences (same stages, error norm and step control).  No
+ * the resident capacity.  mixed = 0 is the FP64 integrator; mixed = 1 solves on the species
+ * block (the momentum / energy rows of a reactor's RHS are zero), with a one-sided species
+ * Jacobian factored in FP32 and every stage solve refined once against the FP64 residual
+ * (same stages, error norm and step control).  No
  * counterpart as one routine in the reference: its rosenbrock_kernel takes the cells of a
  * field, this takes a flat batch. */
 int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,

@@ -1674,6 +1674,289 @@ int orc_rosenbrock_batch(int ns, int nbatch, double *w, double *tg, const double
 }
 
 /* ===================================================================================== */
+/* a9: Rosenbrock-Krylov integrator (kernels_gpu.F90:10360-11205)                         */
+/* ===================================================================================== */
+#define KRY_M 4        /* M_max, index_define.h:27 */
+#define KRY_NSTEP 4    /* nstep, index_define.h:4 */
+#define KM(a, r, c) (a)[((r) - 1) + KRY_M * ((c) - 1)]          /* (KRY_M x KRY_M) */
+
+/* NORM2 as the plain root of the sum of squares in index order */
+static double kry_norm2(int n, const double *v)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + v[i] * v[i];
+    return sqrt(s);
+}
+
+/* rosenbrock_krylov_rhs (kernels_gpu.F90:10794-10976) with the PaSR factor off: the
+ * arithmetic of rosenbrock_rhs on a copy of the state (the species are clipped in the copy
+ * only) with the temperature guess passed by value (intent(in) there) */
+static void kry_rhs(const chem_t *C, const double *w, double t_iter, double *rhs)
+{
+    double tmp[NVMAX], yy[NSMAX];
+    for (int m = 0; m < C->nv; ++m) tmp[m] = w[m];
+    double tg = t_iter;
+    ros_rhs(C, tmp, yy, &tg, rhs);
+}
+
+/* inv_matrix_krylov (:11087-11119): inverse of the leading d x d block in place, through
+ * ludcmp_krylov / lubksb_krylov (the Crout LU of the a8 integrator on that block) */
+static void kry_inv(double *a, int d)
+{
+    if (d == 1) { KM(a, 1, 1) = 1.0 / KM(a, 1, 1); return; }
+    double blk[KRY_M * KRY_M], y[KRY_M * KRY_M];
+    int indx[KRY_M];
+    for (int c = 1; c <= d; ++c)
+        for (int r = 1; r <= d; ++r) blk[(r - 1) + d * (c - 1)] = KM(a, r, c);
+    ludcmp(d, blk, indx);
+    for (int c = 1; c <= d; ++c) {
+        for (int r = 1; r <= d; ++r) y[(r - 1) + d * (c - 1)] = r == c ? 1.0 : 0.0;
+        lubksb(d, blk, indx, &y[d * (c - 1)]);
+    }
+    for (int c = 1; c <= d; ++c)
+        for (int r = 1; r <= d; ++r) KM(a, r, c) = y[(r - 1) + d * (c - 1)];
+}
+
+/* arnoldi (:10664-10792): Krylov basis of the one-sided finite-difference Jacobian at
+ * w_sav started from f(w_sav); stops at the first size kk with
+ * |dt gam H(kk+1,kk)| |lambda(kk)| < 1e-6 and then returns H, the basis Vb and
+ * inv = (I - dt gam H)^-1; when no size up to KRY_M passes, H, Vb and inv stay zero. */
+static void kry_arnoldi(const chem_t *C, const double *w_sav, double dt, double t_iter, const double *fls,
+                        double *Vb, double *inv, double *rhs_sav)
+{
+    const int nv = C->nv;
+    const double gam = 0.572816062482135, TOL = 1.0e-6, eps = 1.0e-08;
+    double HH[(KRY_M + 1) * KRY_M], V[NVMAX * (KRY_M + 1)], Vt[KRY_M * NVMAX], HHinv[KRY_M * KRY_M];
+    double rpr[KRY_M], lambda[KRY_M], jac[NVMAX], wl[NVMAX], rhsp[NVMAX];
+#define HHX(r, c) HH[((r) - 1) + (KRY_M + 1) * ((c) - 1)]
+#define VX(r, c) V[((r) - 1) + (size_t)nv * ((c) - 1)]
+#define VTX(r, c) Vt[((r) - 1) + KRY_M * ((c) - 1)]
+    for (int q = 0; q < (KRY_M + 1) * KRY_M; ++q) HH[q] = 0.0;
+    for (int q = 0; q < nv * (KRY_M + 1); ++q) V[q] = 0.0;
+    for (int q = 0; q < KRY_M * nv; ++q) Vt[q] = 0.0;
+    for (int q = 0; q < KRY_M * KRY_M; ++q) { HHinv[q] = 0.0; inv[q] = 0.0; }
+    for (int q = 0; q < nv * KRY_M; ++q) Vb[q] = 0.0;
+    for (int q = 0; q < KRY_M; ++q) lambda[q] = 0.0;
+    kry_rhs(C, w_sav, t_iter, rhs_sav);
+    if (fls) for (int m = 0; m < nv; ++m) rhs_sav[m] = rhs_sav[m] - fls[m];
+    double nrm = kry_norm2(nv, rhs_sav);
+    for (int m = 1; m <= nv; ++m) VX(m, 1) = rhs_sav[m - 1] / nrm;
+    for (int kk = 1; kk <= KRY_M; ++kk) {
+        /* rosenbrock_krylov_jacobian (:10613-10662) */
+        for (int m = 1; m <= nv; ++m) wl[m - 1] = w_sav[m - 1] + eps * VX(m, kk);
+        kry_rhs(C, wl, t_iter, rhsp);
+        if (fls) for (int m = 0; m < nv; ++m) rhsp[m] = rhsp[m] - fls[m];
+        for (int m = 0; m < nv; ++m) jac[m] = (rhsp[m] - rhs_sav[m]) / eps;
+        for (int jj = 1; jj <= kk; ++jj) {
+            double d = 0.0;
+            for (int m = 1; m <= nv; ++m) d = d + VX(m, jj) * jac[m - 1];
+            HHX(jj, kk) = d;
+            for (int m = 1; m <= nv; ++m) jac[m - 1] = jac[m - 1] - HHX(jj, kk) * VX(m, jj);
+        }
+        HHX(kk + 1, kk) = kry_norm2(nv, jac);
+        for (int m = 1; m <= nv; ++m) VX(m, kk + 1) = jac[m - 1] / HHX(kk + 1, kk);
+        int lim = kk < KRY_M ? kk : KRY_M;
+        for (int ll = 1; ll <= lim; ++ll)
+            for (int jj = 1; jj <= nv; ++jj) VTX(ll, jj) = VX(jj, ll);
+        for (int ll = 1; ll <= KRY_M; ++ll) {
+            rpr[ll - 1] = 0.0;
+            for (int jj = 1; jj <= nv; ++jj) rpr[ll - 1] = rpr[ll - 1] + VTX(ll, jj) * rhs_sav[jj - 1];
+        }
+        for (int n = 1; n <= lim; ++n)
+            for (int nn = 1; nn <= lim; ++nn) {
+                KM(HHinv, n, nn) = -(dt * gam * HHX(n, nn));
+                if (n == nn) KM(HHinv, n, nn) = KM(HHinv, n, nn) + 1.0;
+            }
+        if (kk < KRY_M)
+            for (int n = kk + 1; n <= KRY_M; ++n) KM(HHinv, n, n) = 1.0;
+        kry_inv(HHinv, kk);
+        for (int ll = 1; ll <= KRY_M; ++ll) {
+            lambda[ll - 1] = 0.0;
+            for (int jj = 1; jj <= KRY_M; ++jj) lambda[ll - 1] = lambda[ll - 1] + KM(HHinv, ll, jj) * rpr[jj - 1];
+        }
+        if (fabs(dt * gam * HHX(kk + 1, kk)) * fabs(lambda[kk - 1]) < TOL) {
+            for (int c = 1; c <= kk; ++c)
+                for (int m = 1; m <= nv; ++m) Vb[(m - 1) + (size_t)nv * (c - 1)] = VX(m, c);
+            for (int q = 0; q < KRY_M * KRY_M; ++q) inv[q] = HHinv[q];
+            break;
+        }
+    }
+#undef HHX
+#undef VX
+#undef VTX
+}
+
+/* One cell of rosenbrock_krylov_kernel (:10360-10611) from time_start to time_end:
+ * 4-stage ROK method, a new Krylov basis for every attempted step, error
+ * (yt - ys) / (1e-4 ys + 1e-8) in the 2-norm, dt *= min(5, max(0.2, 0.625 errn^0.1 /
+ * errmax^0.175)).  fls is the cell's transport forcing fl_sav (simpler splitting) or NULL. */
+int orc_rosenbrock_krylov_cell(const chem_t *C, double *vec, double t_iter, double dttry, double time_start,
+                               double time_end, int maxsteps, const double *fls, int *nsteps)
+{
+    const int nv = C->nv, ns = C->ns;
+    const double gam = 0.572816062482135;
+    double alpha[5][4] = {{0}}, gamma_c[5][4] = {{0}}, bs[5], bt[5];
+    alpha[2][1] = 0.432364435748567;
+    alpha[3][1] = -0.514211316876170; alpha[3][2] = 1.382271144617360;
+    alpha[4][1] = -0.514211316876170; alpha[4][2] = 1.382271144617360; alpha[4][3] = 0.0;
+    bs[1] = 0.194335256262729; bs[2] = 0.483167813989227; bs[3] = 0.0; bs[4] = 0.322496929748044;
+    gamma_c[2][1] = -0.602765307997356;
+    gamma_c[3][1] = -1.389195789724843; gamma_c[3][2] = 1.072950969011413;
+    gamma_c[4][1] = 0.992356412977094; gamma_c[4][2] = -1.390032613873701; gamma_c[4][3] = -0.440875890223325;
+    bt[1] = -0.217819895945721; bt[2] = 1.03130847478467; bt[3] = 0.186511421161047; bt[4] = 0.0;
+    double Vb[NVMAX * KRY_M], inv[KRY_M * KRY_M], rhs_sav[NVMAX];
+    double A[NVMAX * KRY_M], Bm[NVMAX * NVMAX], Ks[NVMAX * (KRY_NSTEP + 1)];
+    double ys[NVMAX], yt[NVMAX], F[NVMAX], GK[NVMAX], Cv[NVMAX], errv[NVMAX], yy[NSMAX];
+#define VBX(r, c) Vb[((r) - 1) + (size_t)nv * ((c) - 1)]
+#define AX(r, c) A[((r) - 1) + (size_t)nv * ((c) - 1)]
+#define BX(r, c) Bm[((r) - 1) + (size_t)nv * ((c) - 1)]
+#define KSX(r, c) Ks[((r) - 1) + (size_t)nv * ((c) - 1)]
+    double time = time_start, dt = dttry, errn = 1.0;
+    int it = 0;
+    while (time < (time_end - 1.0e-12)) {
+        dt = dt < time_end - time ? dt : time_end - time;
+        it = it + 1;
+        kry_arnoldi(C, vec, dt, t_iter, fls, Vb, inv, rhs_sav);
+        /* A = Vb (I - inv), B = A Vb^T */
+        for (int kk = 1; kk <= KRY_M; ++kk)
+            for (int ll = 1; ll <= nv; ++ll) {
+                double a = VBX(ll, 1) * ((kk == 1 ? 1.0 : 0.0) - KM(inv, 1, kk));
+                for (int jj = 2; jj <= KRY_M; ++jj) a = a + VBX(ll, jj) * ((jj == kk ? 1.0 : 0.0) - KM(inv, jj, kk));
+                AX(ll, kk) = a;
+            }
+        for (int kk = 1; kk <= nv; ++kk)
+            for (int ll = 1; ll <= nv; ++ll) {
+                double b = AX(ll, 1) * VBX(kk, 1);
+                for (int jj = 2; jj <= KRY_M; ++jj) b = b + AX(ll, jj) * VBX(kk, jj);
+                BX(ll, kk) = b;
+            }
+        for (int ii = 1; ii <= KRY_NSTEP; ++ii) {
+            for (int m = 1; m <= nv; ++m) ys[m - 1] = vec[m - 1];
+            for (int jj = 1; jj <= ii - 1; ++jj)
+                for (int m = 1; m <= nv; ++m) ys[m - 1] = ys[m - 1] + dt * alpha[ii][jj] * KSX(m, jj);
+            kry_rhs(C, ys, t_iter, F);
+            for (int m = 0; m < nv; ++m) GK[m] = 0.0;
+            for (int jj = 1; jj <= ii - 1; ++jj)
+                for (int m = 1; m <= nv; ++m) GK[m - 1] = GK[m - 1] + gamma_c[ii][jj] / gam * KSX(m, jj);
+            for (int m = 0; m < nv; ++m) F[m] = F[m] + GK[m];
+            if (fls) for (int m = 0; m < nv; ++m) F[m] = F[m] - fls[m];
+            for (int ll = 1; ll <= nv; ++ll) {
+                double c = BX(ll, 1) * F[0];
+                for (int jj = 2; jj <= nv; ++jj) c = c + BX(ll, jj) * F[jj - 1];
+                Cv[ll - 1] = c;
+            }
+            for (int m = 1; m <= nv; ++m) KSX(m, ii) = F[m - 1] - Cv[m - 1] - GK[m - 1];
+        }
+        for (int ll = 1; ll <= nv; ++ll) {
+            ys[ll - 1] = vec[ll - 1];
+            yt[ll - 1] = vec[ll - 1];
+            for (int ii = 1; ii <= KRY_NSTEP; ++ii) {
+                ys[ll - 1] = ys[ll - 1] + dt * bs[ii] * KSX(ll, ii);
+                yt[ll - 1] = yt[ll - 1] + dt * bt[ii] * KSX(ll, ii);
+            }
+            errv[ll - 1] = (yt[ll - 1] - ys[ll - 1]) / ((1.0e-4) * ys[ll - 1] + 1.0e-8);
+        }
+        double errmax = kry_norm2(nv, errv);
+        double fac = 0.625 * pow(errn, 0.1) / pow(errmax, 0.175);
+        fac = fac > 0.2 ? fac : 0.2;
+        fac = fac < 5.0 ? fac : 5.0;
+        double dtnext = dt * fac;
+        if (errmax < 1.0) {
+            time = time + dt;
+            for (int m = 0; m < nv; ++m) vec[m] = ys[m];
+            double rho = vec[0];
+            for (int n = 2; n <= ns; ++n) rho = rho + vec[n - 1];
+            double ri = 1.0 / rho;
+            for (int n = 0; n < ns; ++n) yy[n] = vec[n] * ri;
+            double uu = vec[I_U - 1] * ri, vv = vec[I_V - 1] * ri, ww = vec[I_W - 1] * ri;
+            double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            double ee = vec[I_E - 1] * ri - qq;
+            t_iter = temperature_from_e(ns, ee, t_iter, C->cv_coeff, C->nsetcv, C->trange, C->indx_cp_l,
+                                        C->indx_cp_r, C->tol_iter_nr, yy);
+            dt = dtnext;
+            errn = errmax;
+        } else {
+            dt = dtnext;
+            errn = errmax;
+        }
+        if (it > maxsteps) break;
+    }
+    if (nsteps) *nsteps = it;
+#undef VBX
+#undef AX
+#undef BX
+#undef KSX
+    return 0;
+}
+
+int orc_rosenbrock_krylov(int ns, int nv, int nv_aux, int nx, int ny, int nz, int ng,
+                          double *w, double *w_aux, const double *fl, const double *mw,
+                          int nreactions, const double *arr_a, const double *arr_b,
+                          const double *arr_ea, const double *tb_eff,
+                          const double *falloff_coeffs, const int *reac_ty, const int *isrev,
+                          const double *r_coeffs, const double *p_coeffs,
+                          const double *kc_tab, int num_t_tab, double t_min_tab,
+                          double dt_tab, double r_univ, double tol_iter_nr,
+                          const double *cv_coeff, int nsetcv, const double *trange,
+                          int indx_cp_l, int indx_cp_r, double dttry, double time_start,
+                          double time_end, int simpler_splitting, const double *fl_sav, int maxsteps,
+                          int enable_pasr, double les_c_yoshi, double les_c_mix, double les_c_eps,
+                          const double *dcsidx, const double *detady, const double *dzitdz, void *stream)
+{
+    (void)nv_aux; (void)fl; (void)les_c_yoshi; (void)les_c_mix; (void)les_c_eps;
+    (void)dcsidx; (void)detady; (void)dzitdz; (void)stream;
+    if (enable_pasr != 0) return -3;
+    if (simpler_splitting == 1 && fl_sav == NULL) return -1;
+    chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
+                falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
+                tol_iter_nr, cv_coeff, trange};
+    const size_t fs = (size_t)nx * ny * nz;
+    OMP_FOR
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        double zb = W(w_aux, i, j, k, J_Z);
+        if (!(zb > 5.0e-4 && zb < 1.0 - 5.0e-4)) continue;
+        double vec[NVMAX], fls[NVMAX];
+        for (int m = 1; m <= nv; ++m) vec[m - 1] = W(w, i, j, k, m);
+        if (simpler_splitting == 1)
+            for (int m = 1; m <= nv; ++m)
+                fls[m - 1] = fl_sav[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)) + fs * (m - 1)];
+        orc_rosenbrock_krylov_cell(&C, vec, W(w_aux, i, j, k, J_T), dttry, time_start, time_end, maxsteps,
+                                   simpler_splitting == 1 ? fls : NULL, NULL);
+        for (int m = 1; m <= nv; ++m) W(w, i, j, k, m) = vec[m - 1];
+    }
+    return 0;
+}
+
+/* Batched interface of the Krylov integrator over a flat list of reactors (state
+ * w[nv][nbatch] SoA, temperature guess tg[nbatch], left as given), for the pinning tests. */
+int orc_rosenbrock_krylov_batch(int ns, int nbatch, double *w, const double *tg, const double *mw, int nreactions,
+                                const double *arr_a, const double *arr_b, const double *arr_ea,
+                                const double *tb_eff, const double *falloff_coeffs, const int *reac_ty,
+                                const int *isrev, const double *r_coeffs, const double *p_coeffs,
+                                const double *kc_tab, int num_t_tab, double t_min_tab, double dt_tab,
+                                double r_univ, double tol_iter_nr, const double *cv_coeff, int nsetcv,
+                                const double *trange, int indx_cp_l, int indx_cp_r, const double *dttry,
+                                const double *time_end, int maxsteps, int *nsteps)
+{
+    int nv = ns + 4;
+    chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
+                falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
+                tol_iter_nr, cv_coeff, trange};
+    OMP_FOR
+    for (int b = 0; b < nbatch; ++b) {
+        double wv[NVMAX];
+        for (int m = 0; m < nv; ++m) wv[m] = w[(size_t)m * nbatch + b];
+        int it = 0;
+        orc_rosenbrock_krylov_cell(&C, wv, tg[b], dttry[b], 0.0, time_end[b], maxsteps, NULL, &it);
+        for (int m = 0; m < nv; ++m) w[(size_t)m * nbatch + b] = wv[m];
+        if (nsteps) nsteps[b] = it;
+    }
+    return 0;
+}
+
+/* ===================================================================================== */
 /* a10, a12: RK bookkeeping and time step                                                 */
 /* ===================================================================================== */
 int orc_init_flux(int nx, int ny, int nz, int nv, double *fl, double *fln, double rhodt, void *stream)

@@ -38,6 +38,8 @@ _ARGS = {
     "compute_chemistry": "iiiiiiii" "ppp" "p" "pppp" "p" "pp" "ppp" "i" "ddd" "ii" "p" "i" "p" "d" "s",
     "rosenbrock": "iiiiiii" "ppp" "p" "i" "pppp" "p" "pp" "ppp" "i" "dddd" "p" "i" "p" "ii" "ddd" "iid" "ii"
                   "ddd" "ppp" "s",
+    "rosenbrock_krylov": "iiiiiii" "ppp" "p" "i" "pppp" "p" "pp" "ppp" "i" "dddd" "p" "i" "p" "ii" "ddd" "i" "p"
+                         "ii" "ddd" "ppp" "s",
     "init_flux": "iiii" "pp" "d" "s",
     "update_flux": "iiii" "pp" "d" "s",
     "update_field": "iiiii" "ppp" "s",

@@ -87,6 +87,7 @@ class Numerics:
     ros_eps: float = 1e-8
     ros_maxsteps: int = 10000
     ros_maxtry: int = 40
+    rosenbrock_version: int = 1      # 1: ROS4 (rosenbrock), 2: Rosenbrock-Krylov
 
 
 class Solver:
@@ -252,6 +253,17 @@ class Solver:
                      tb.indx_cp_r, dt_chem, self.time, self.time + dt_chem, n.ros_maxsteps, n.ros_maxtry, n.ros_eps,
                      0, 0, 0.0, 0.0, 0.0, self.dxi[0], self.dxi[1], self.dxi[2], self.be.stream)
 
+    def rosenbrock_krylov(self, dt_chem: float, fl_sav=None):
+        """rosenbrock_krylov (multideal_gpu.F90:769-838); fl_sav given: simpler splitting,
+        the transport forcing subtracted from every chemistry right-hand side."""
+        tb, n = self.tb, self.num
+        self.be.call("rosenbrock_krylov", self.ns, self.nv, self.nv_aux, self.nx, self.ny, self.nz, self.ng,
+                     self.w, self.w_aux, self.fl, self.t_mw, self.nr, self.t_arr_a, self.t_arr_b, self.t_arr_ea,
+                     self.t_tb, self.t_fall, self.t_rty, self.t_rev, self.t_rco, self.t_pco, self.t_kc, tb.num_t_tab,
+                     tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l,
+                     tb.indx_cp_r, dt_chem, self.time, self.time + dt_chem, int(fl_sav is not None), fl_sav,
+                     n.ros_maxsteps, 0, 0.0, 0.0, 0.0, self.dxi[0], self.dxi[1], self.dxi[2], self.be.stream)
+
     def update_ghost(self):
         """Physical boundary conditions of the non-periodic faces (update_ghost, do_swap=0)."""
         for ilat, tag in enumerate(self.bctags, start=1):
@@ -347,10 +359,11 @@ class Solver:
         """advance_solution (multideal_gpu.F90:1853-1931) without LES / energy deposition."""
         if self.num.enable_chemistry == 2:
             dt_chem = 0.5 * self.dt
-            self.rosenbrock(dt_chem)
+            chem = self.rosenbrock if self.num.rosenbrock_version == 1 else self.rosenbrock_krylov
+            chem(dt_chem)
             self.rk_sync(fused=False)
             self.compute_aux(central=1, ghost=0)
-            self.rosenbrock(dt_chem)
+            chem(dt_chem)
         else:
             self.rk_sync(fused=fused)
         self.time += self.dt

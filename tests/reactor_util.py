@@ -93,3 +93,36 @@ def t_from_e(tables, e, y, lo=200.0, hi=6000.0):
     inversion independent of the Newton iteration under test."""
     from scipy.optimize import brentq
     return brentq(lambda t: tables.e_from_T(t, y) - e, lo, hi, xtol=1e-12, rtol=1e-15, maxiter=200)
+
+
+class KrylovBatch:
+    """The oracle's Rosenbrock-Krylov restatement (orc_rosenbrock_krylov_batch, following
+    rosenbrock_krylov_kernel, kernels_gpu.F90:10360-10611) over a flat batch of reactors."""
+
+    def __init__(self, lib, tables, maxsteps=10000):
+        self.rb = ReactorBatch(lib, tables)
+        self.lib, self.tb, self.maxsteps = self.rb.lib, tables, maxsteps
+        fn = self.lib.orc_rosenbrock_krylov_batch
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        fn.restype = I
+        fn.argtypes = [I, I, P, P, P, I] + [P] * 10 + [I, D, D, D, D, P, I, P, I, I, P, P, I, P]
+        self.fn = fn
+
+    def integrate(self, w, tg, dttry, time_end):
+        """w (nv, nb) in place from t = 0 to time_end with first step dttry; tg is the
+        temperature guess (not updated).  Returns the attempted steps per reactor."""
+        t = self.tb
+        nb = w.shape[1]
+        assert w.flags.c_contiguous and w.dtype == np.float64
+        tg = np.ascontiguousarray(np.broadcast_to(np.asarray(tg, dtype=np.float64), (nb,)))
+        dttry = np.broadcast_to(np.asarray(dttry, dtype=np.float64), (nb,)).copy()
+        tend = np.broadcast_to(np.asarray(time_end, dtype=np.float64), (nb,)).copy()
+        steps = np.zeros(nb, dtype=np.int32)
+        k = self.rb._keep
+        rc = self.fn(t.n_species, nb, w.ctypes.data, tg.ctypes.data, k[0].ctypes.data, t.nreactions,
+                     *(a.ctypes.data for a in k[1:11]), t.num_t_tab, t.t_min_tab, t.dt_tab, R_UNIV, TOL_ITER_NR,
+                     k[11].ctypes.data, t.nsetcv, k[12].ctypes.data, t.indx_cp_l, t.indx_cp_r, dttry.ctypes.data,
+                     tend.ctypes.data, self.maxsteps, steps.ctypes.data)
+        assert rc == 0
+        return steps
+

@@ -276,6 +276,37 @@ def test_rosenbrock_cells(pkg, h2_tables, hip_backend, cpu_backend):
     assert err.max() < 1e-9, err
 
 
+def test_rosenbrock_krylov_cells(pkg, h2_tables, hip_backend, cpu_backend):
+    """Rosenbrock-Krylov chemistry (rosenbrock_krylov_kernel) over one time step on a small
+    box, plain and with the simpler-splitting transport forcing fl_sav.  With the forcing the
+    method's accept/reject and Krylov-size decisions make it sensitive to rounding (the
+    oracle itself moves by ~1e-5 when its input moves by 1e-15): there the bound is ten
+    times the oracle's own response to that perturbation."""
+    num = pkg.solver.Numerics(enable_chemistry=2, rosenbrock_version=2)
+    for forcing in (False, True):
+        g, c = make_pair(pkg, h2_tables, hip_backend, cpu_backend, shape=(6, 5, 4), numerics=num,
+                         temperature=1500.0, perturb_y=0.4)[:2]
+        w0 = c.w.copy()
+        f = None
+        if forcing:
+            # a transport-like forcing: moves each variable by ~0.1 % of itself over the step
+            rng = np.random.default_rng(5)
+            f = rng.standard_normal(c.fl.shape) * 1e-3 * np.abs(interior(c.w, NG)) / 2e-6
+        for s, fs in ((g, g.be.asarray(f) if forcing else None), (c, f)):
+            s.compute_aux()
+            s.rosenbrock_krylov(2e-6, fs)
+        g.be.sync()
+        err = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
+        bound = 1e-9
+        if forcing:
+            ref = c.w.copy()
+            c.w[...] = w0 * (1.0 + 1e-15)
+            c.compute_aux()
+            c.rosenbrock_krylov(2e-6, f)
+            bound = max(bound, 10.0 * rel_maxnorm(interior(c.w, NG), interior(ref, NG)).max())
+        assert err.max() < bound, (forcing, bound, err)
+
+
 # ---------------------------------------------------------------- fused substep stages
 def _fused_vs_sequence(pkg, tables, hip_backend, cpu_backend, shape=SHAPE, numerics=None, bctags=(0,) * 6,
                        steps=1, **kw):
