@@ -215,6 +215,40 @@ def serving_bench(pkg, duration: float, rate: float = 200.0):
     return st
 
 
+def c4_bench(pkg, shape=(512, 256, 128), steps: int = 2, warmup: int = 1):
+    """The 14-species Keromnes-2013 syngas mixing layer on one rank's 512 x 256 x 128 box:
+    RK3 steps of the fused path with the sensor threshold set so that about 10 % of the
+    x faces take the WENO flux (WENO5, Roe splitting), cell-updates/s."""
+    import torch
+    tb = pkg.cases.keromnes_mechanism().tables()
+    nx, ny, nz = shape
+    print(f"[bench] building the {nx}x{ny}x{nz} syngas mixing layer", file=sys.stderr, flush=True)
+    w, tg, info = pkg.cases.mixing_layer(tb, nx, ny, nz, seed=2024)
+    be = pkg.solver.HipBackend("cuda:0")
+    num = pkg.solver.Numerics(weno_scheme=3)
+    s = pkg.solver.Solver(be, tb, nx, ny, nz, 3, info["length"], numerics=num, rho0=info["rho0"], u0=info["u0"])
+    s.set_state(w, tg)
+    del w, tg
+    s.compute_aux()
+    duc = be.to_numpy(s.w_aux[tb.n_species + 8])[3:-3, 3:-3, 3:-3]
+    num.sensor_threshold = pkg.cases.weno_threshold_for_fraction(duc, num.weno_scheme, 0.10)
+    del duc
+    s.compute_dt(0.5)
+    for _ in range(warmup):
+        s.rk_sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.rk_sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": f"{nx}x{ny}x{nz} H2/CO-air temporal mixing layers, Keromnes-2013 (14 species, "
+                        f"37 reactions), M_c 0.8, WENO5 on ~10 % of the faces, KEEP elsewhere",
+            "cells": nx * ny * nz, "steps": steps, "ms_per_step": 1e3 * el / steps,
+            "cell_updates_per_s": 3 * nx * ny * nz * steps / el, "sensor_threshold": num.sensor_threshold,
+            "dt_s": s.dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +262,8 @@ def main():
                     help="also time the batched chemistry on this many reactors (e.g. 2097152)")
     ap.add_argument("--serving", type=float, default=0.0,
                     help="also run the reactor server on a Poisson stream of this many seconds")
+    ap.add_argument("--c4", action="store_true",
+                    help="also time the 512x256x128 14-species syngas mixing layer with ~10 %% WENO faces")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (5 times): the known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -316,6 +352,8 @@ def main():
     if args.serving > 0:
         print(f"[bench] reactor server, {args.serving} s stream", file=sys.stderr, flush=True)
         out["serving"] = serving_bench(pkg, args.serving)
+    if args.c4:
+        out["c4_mixing_layer"] = c4_bench(pkg)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pkg, tables, args.cpu_n, args.cpu_steps)

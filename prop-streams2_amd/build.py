@@ -35,6 +35,7 @@ HIPCC_FLAGS = [
 # through it; without the hoist the species kernel fits 6 waves per SIMD instead of 4.
 FILE_FLAGS = {
     "finish.hip": ["-mllvm", "-disable-machine-licm"],
+    "euler.hip": ["-ffp-contract=off"],
 }
 
 

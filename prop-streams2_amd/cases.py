@@ -11,6 +11,12 @@
   seeded Dirichlet(1) over the species, rho from the equation of state, at rest.
 * ``shock_tube`` — the reactive shock-tube initial state of init_reactive_tube
   (multideal.F90:5305-5381) extruded along x.
+* ``mixing_layer`` — syngas / air temporal mixing layers on a periodic box (the 14-species
+  Keromnes-2013 mechanism): a fuel band (H2:CO = 1:1 molar, 300 K) between two tanh
+  interfaces in y, hot air (1500 K) outside, velocity jump from a convective Mach number
+  M_c = dU / (c_fuel + c_air), uniform pressure, and a seeded 1 % broadband (solenoidal)
+  velocity perturbation.  Two interfaces make the y direction periodic, so the one-rank box
+  needs no inflow or outflow treatment.
 
 Arrays are returned as ``w[m, k, j, i]`` (C order, i.e. the reference's Fortran layout
 ``w(i, j, k, m)``) including the ng ghost layers, which are filled with the periodic
@@ -48,12 +54,15 @@ def _pad_periodic(a: np.ndarray, ng: int) -> np.ndarray:
     return np.pad(a, [(0, 0)] + [(ng, ng)] * (a.ndim - 1), mode="wrap")
 
 
-def solenoidal_field(n: int, seed: int, k0: float = 4.0) -> np.ndarray:
-    """Random-phase solenoidal velocity (3, n, n, n) with unit rms speed sqrt(<u.u>)=1."""
+def solenoidal_field(n, seed: int, k0: float = 4.0) -> np.ndarray:
+    """Random-phase solenoidal velocity (3, nz, ny, nx) with unit rms speed sqrt(<u.u>)=1,
+    periodic on the box; n is the cube edge or the shape (nz, ny, nx)."""
+    nz, ny, nx = (n, n, n) if np.isscalar(n) else tuple(n)
     rng = np.random.default_rng(seed)
-    k1 = np.fft.fftfreq(n, d=1.0 / n)
-    kr = np.fft.rfftfreq(n, d=1.0 / n)
-    kz, ky, kx = np.meshgrid(k1, k1, kr, indexing="ij")
+    kz1 = np.fft.fftfreq(nz, d=1.0 / nz)
+    ky1 = np.fft.fftfreq(ny, d=1.0 / ny)
+    kr = np.fft.rfftfreq(nx, d=1.0 / nx)
+    kz, ky, kx = np.meshgrid(kz1, ky1, kr, indexing="ij")
     kk = np.sqrt(kx * kx + ky * ky + kz * kz)
     kk[0, 0, 0] = 1.0
     spec = (kk / k0) ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
@@ -64,7 +73,7 @@ def solenoidal_field(n: int, seed: int, k0: float = 4.0) -> np.ndarray:
     # project onto the plane normal to k
     kdot = (kx * xi[0] + ky * xi[1] + kz * xi[2]) / (kk * kk)
     uh = [xi[0] - kx * kdot, xi[1] - ky * kdot, xi[2] - kz * kdot]
-    u = np.stack([np.fft.irfftn(c, s=(n, n, n), axes=(0, 1, 2)) for c in uh])    # axes (k, j, i)
+    u = np.stack([np.fft.irfftn(c, s=(nz, ny, nx), axes=(0, 1, 2)) for c in uh])    # axes (k, j, i)
     u -= u.mean(axis=(1, 2, 3), keepdims=True)
     urms = math.sqrt(float((u * u).sum(axis=0).mean()))
     return u / urms
@@ -140,3 +149,67 @@ def shock_tube(tables: Tables, nx: int, init_y: Sequence[float], rho0: float, p0
     wp = np.pad(w, [(0, 0), (ng, ng), (ng, ng), (ng, ng)], mode="edge")
     tg = np.where(np.pad(np.broadcast_to(left, (nz, ny, nx)), [(ng, ng)] * 3, mode="edge"), t0, t_t)
     return wp, tg, dict(t0=t0, t_tube=t_t, rho0=rho0, p0=p0)
+
+
+def keromnes_mechanism() -> Mechanism:
+    return Mechanism(str(mechanism_path("keromnes2013_14sp.yaml")))
+
+
+def mixing_layer(tables: Tables, nx: int, ny: int, nz: int, seed: int = 2024, ng: int = 3, dx: float = 1.0e-5,
+                 mach_c: float = 0.8, t_fuel: float = 300.0, t_air: float = 1500.0, pressure: float = 101325.0,
+                 thickness: float = 1.0 / 40.0, perturbation: float = 0.01):
+    """Conserved variables (nv, nz+2ng, ny+2ng, nx+2ng), the temperature guess and the box
+    lengths of the double temporal mixing layer; the band's fuel fraction is
+    phi(y) = (tanh((y - Ly/4)/d) - tanh((y - 3Ly/4)/d)) / 2 with d = thickness * Ly, and Y and
+    T are the phi-weighted means of the two streams."""
+    y_fuel = mole_to_mass(tables, {"H2": 1.0, "CO": 1.0})
+    y_air = mole_to_mass(tables, {"O2": 1.0, "N2": 3.76})
+
+    def sound(y, t):
+        r = tables.rmix(y)
+        cp = tables.cp_from_T(t, y)
+        return math.sqrt(cp / (cp - r) * r * t)
+
+    du = mach_c * (sound(y_fuel, t_fuel) + sound(y_air, t_air))
+    ly = ny * dx
+    yc = (np.arange(ny) + 0.5) * dx
+    d = thickness * ly
+    phi = 0.5 * (np.tanh((yc - 0.25 * ly) / d) - np.tanh((yc - 0.75 * ly) / d))
+    ns = tables.n_species
+    Y = phi[:, None] * y_fuel[None, :] + (1.0 - phi[:, None]) * y_air[None, :]          # (ny, ns)
+    T = phi * t_fuel + (1.0 - phi) * t_air
+    rho = pressure / ((Y @ tables.rgas) * T)
+    e = tables.e_from_T_many(T, Y)
+    u = du * (phi - 0.5)
+    # broadband perturbation: a solenoidal field on the cube that holds the box, cropped
+    pert = solenoidal_field((nz, ny, nx), seed) * (perturbation * du)
+    vel = pert.copy()
+    vel[0] += u[None, :, None]
+    w = np.empty((ns + 4, nz, ny, nx))
+    for s in range(ns):
+        w[s] = (rho * Y[:, s])[None, :, None]
+    rb = rho[None, :, None]
+    w[ns + 0] = rb * vel[0]
+    w[ns + 1] = rb * vel[1]
+    w[ns + 2] = rb * vel[2]
+    w[ns + 3] = rb * (e[None, :, None] + 0.5 * (vel * vel).sum(axis=0))
+    wp = _pad_periodic(w, ng)
+    tg = np.pad(np.broadcast_to(T[None, :, None], (nz, ny, nx)), [(ng, ng)] * 3, mode="wrap")
+    info = dict(rho0=float(rho.mean()), u0=du, length=[nx * dx, ny * dx, nz * dx], du=du)
+    return wp, np.ascontiguousarray(tg), info
+
+
+def weno_threshold_for_fraction(ducros: np.ndarray, ws: int, fraction: float = 0.10) -> float:
+    """Sensor threshold at which about `fraction` of the x faces take WENO: face i+1/2 is
+    shocked when a cell of its 2*ws-wide stencil has Ducros above the threshold
+    (shock sensing of the euler kernels).  ducros: interior (nz, ny, nx), periodic in x."""
+    win = np.max(np.stack([np.roll(ducros, -s, axis=2) for s in range(-ws + 1, ws + 1)]), axis=0)
+    q = float(np.quantile(win, 1.0 - fraction))
+    # midway between two distinct sensor values, so that no cell sits on the threshold (a
+    # cell exactly at it would flip with the last bit of its sensor value)
+    u = np.unique(win)
+    i = int(np.searchsorted(u, q))
+    if 0 < i < u.size:
+        return float(0.5 * (u[i - 1] + u[i]))
+    return q
+
