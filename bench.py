@@ -197,7 +197,7 @@ def reactor_bench(pkg, n: int, reps: int = 3):
     return out
 
 
-def serving_bench(pkg, duration: float, rate: float = 200.0):
+def serving_bench(pkg, duration: float, rate: float = 200.0, budget: int = 200):
     """Poisson stream (seeded, `rate` req/s for `duration` s) of constant-pressure reactor
     requests, 1-4096 reactors each (log-uniform), H2-air (9 species) and H2/CO syngas
     (Keromnes 14 species), T0 900-1800 K, phi 0.3-3 (log-uniform), horizon 1e-5..1e-2 s
@@ -207,10 +207,11 @@ def serving_bench(pkg, duration: float, rate: float = 200.0):
     fuels = {"h2": {"H2": 1.0}, "syngas": {"H2": 0.5, "CO": 0.5}}
     reqs = pkg.serving.poisson_requests(mechs, rate=rate, duration=duration, seed=2024, fuels=fuels)
     be = pkg.solver.HipBackend("cuda:0")
-    srv = pkg.serving.ReactorServer(be, mechs, hbm_bytes=8 << 30, slices=10, mixed=True)
+    srv = pkg.serving.ReactorServer(be, mechs, hbm_bytes=8 << 30, slices=10, mixed=True, budget=budget)
     st = srv.run(reqs, realtime=True, max_seconds=240)
     st.pop("completed")
-    st.update({"arrival_rate_per_s": rate, "stream_s": duration, "submitted": len(reqs),
+    st.update({"arrival_rate_per_s": rate, "stream_s": duration, "step_budget_per_tick": budget,
+               "submitted": len(reqs),
                "submitted_reactors": int(sum(r.n for r in reqs))})
     return st
 

@@ -284,7 +284,10 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
  *     h) with h the mixture enthalpy; T from h by Newton on cp_coeff, rho = p/(R T)
  *     (rgas and cp_coeff are then required).  The reference's reactor is constant volume.
  * tguess[b] is the Newton start in and the last temperature out; nacc/nrej (may be NULL)
- * receive the accepted / rejected step counts.  Kinetics come from the device copy of
+ * receive the accepted / rejected step counts.  t_done/dt_next (both NULL, or both given)
+ * make the call resumable with a step budget: reactor b starts at t_done[b] with the step
+ * dt_next[b] (dttry[b] if not positive), takes at most maxsteps steps and returns the time
+ * reached (time_end[b] when done) and the step it would take next.  Kinetics come from the device copy of
  * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
  * counter (zeroed by the call on the stream); wg_per_cu <= 0 sizes the persistent grid to
  * the resident capacity.  mixed = 0 is the FP64 integrator; mixed = 1 solves on the species
@@ -300,7 +303,8 @@ int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const 
                           double t_min_tab, double dt_tab, double r_univ, double tol_iter_nr,
                           const double *cv_coeff, int nsetcv, const double *trange, int indx_cp_l,
                           int indx_cp_r, int maxsteps, int maxtry, double tol, int *nacc, int *nrej,
-                          unsigned *work, int wg_per_cu, int mixed, void *stream);
+                          double *t_done, double *dt_next, unsigned *work, int wg_per_cu, int mixed,
+                          void *stream);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,7 @@ _ARGS = {
     "compute_dt": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
     "rhs_sweep": "iiiiiiiii" "ppppp" "p" "iiii" "d" "pp" "i" "p" "ii" "p" "ddd" "ii" "d" "i" "d" "pp" "s",
     "rhs_finish": "iiiiiii" "pppppppp" "iii" "ppp" "i" "dddd" "p" "s",
-    "reactor_batch": "ii" "pppp" "ppp" "p" "i" "pp" "i" "dddd" "p" "i" "p" "ii" "ii" "d" "pp" "p" "ii" "s",
+    "reactor_batch": "ii" "pppp" "ppp" "p" "i" "pp" "i" "dddd" "p" "i" "p" "ii" "ii" "d" "pp" "pp" "p" "ii" "s",
 }
 
 SIGNATURES: Dict[str, str] = dict(_ARGS)

@@ -197,8 +197,14 @@ __device__ void lubksb(const T *a, const int *indx, T *bv) {
 #undef A_
 }
 
+// Step counts out; for a resumable integration (resume = true, the batched reactors with a
+// step budget) the first step is clipped to the interval like every later one, and the
+// time reached, the next step size and whether time_end was reached come back.
 struct RosStats {
   int nacc = 0, nrej = 0;
+  bool resume = false;
+  int done = 0;
+  double t_end = 0.0, dt_next = 0.0;
 };
 
 // One cell from time_start to time_end (rosenbrock_kernel body); rhs(wv, tg, out) is the
@@ -219,6 +225,7 @@ __device__ void rosenbrock_cell_f(const Rhs &rhs_fn, double *wv, double &tg, dou
   int indx[NV];
   int finish = 0;
   double time = time_start, dt = dttry, itol = 1.0 / tol, dtnext = dt;
+  if (st.resume && dt * 1.5 > time_end - time) { dt = time_end - time; finish = 1; }
   for (int iter = 1; iter <= maxsteps; ++iter) {
     const double t0 = time;
     for (int n = 0; n < NV; ++n) {
@@ -276,10 +283,12 @@ __device__ void rosenbrock_cell_f(const Rhs &rhs_fn, double *wv, double &tg, dou
     }
     st.nacc += 1;
     const double remaining = time_end - time;
-    if (finish == 1 || remaining < 1.0e-12 * time_end) break;
+    if (finish == 1 || remaining < 1.0e-12 * time_end) { st.done = 1; break; }
     if (dtnext * 1.5 > remaining) { dt = remaining; finish = 1; }
     else { dt = dtnext; finish = 0; }
   }
+  st.t_end = time;
+  st.dt_next = st.done ? dtnext : dt;
 }
 
 // The reference-form right-hand side (rosenbrock_rhs) bound to its tables.

@@ -145,6 +145,7 @@ __device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, do
   int indx[NS];
   int finish = 0;
   double time = time_start, dt = dttry, itol = 1.0 / tol, dtnext = dt;
+  if (st.resume && dt * 1.5 > time_end - time) { dt = time_end - time; finish = 1; }
   for (int iter = 1; iter <= maxsteps; ++iter) {
     const double t0 = time;
     rhs_fn(wv, tg, rhs_sav);
@@ -202,16 +203,19 @@ __device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, do
     }
     st.nacc += 1;
     const double remaining = time_end - time;
-    if (finish == 1 || remaining < 1.0e-12 * time_end) break;
+    if (finish == 1 || remaining < 1.0e-12 * time_end) { st.done = 1; break; }
     if (dtnext * 1.5 > remaining) { dt = remaining; finish = 1; }
     else { dt = dtnext; finish = 0; }
   }
+  st.t_end = time;
+  st.dt_next = st.done ? dtnext : dt;
 }
 
 struct ReactorArgs {
   double *w, *tg;                 // w[m * nbatch + b] (SoA), T guess / result
   const double *dttry, *time_end; // per reactor
   int *nacc, *nrej;               // per reactor (may be null)
+  double *t_done, *dt_next;       // per reactor, resumable mode (both null: from t = 0)
   unsigned *next;                 // work counter (starts at 0)
   int nbatch, maxsteps, maxtry, mixed;
   double tol, tol_iter_nr;
@@ -230,13 +234,28 @@ __device__ inline void integrate_one(const Rhs &f, const ReactorArgs &A, unsigne
   for (int m = 0; m < NV; ++m) wv[m] = A.w[(long)m * nb + r];
   double tg = A.tg[r];
   RosStats st;
-  if (MIXED) rosenbrock_cell_mp<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
-  else rosenbrock_cell_f<NS>(f, wv, tg, A.dttry[r], 0.0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+  double t0 = 0.0, dt0 = A.dttry[r];
+  if (A.t_done) {
+    st.resume = true;
+    t0 = A.t_done[r];
+    if (A.dt_next[r] > 0.0) dt0 = A.dt_next[r];
+    if (t0 >= A.time_end[r]) {                        // finished earlier: nothing to do
+      if (A.nacc) A.nacc[r] = 0;
+      if (A.nrej) A.nrej[r] = 0;
+      return;
+    }
+  }
+  if (MIXED) rosenbrock_cell_mp<NS>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+  else rosenbrock_cell_f<NS>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
 #pragma unroll
   for (int m = 0; m < NV; ++m) A.w[(long)m * nb + r] = wv[m];
   A.tg[r] = tg;
   if (A.nacc) A.nacc[r] = st.nacc;
   if (A.nrej) A.nrej[r] = st.nrej;
+  if (A.t_done) {
+    A.t_done[r] = st.done ? A.time_end[r] : st.t_end;
+    A.dt_next[r] = st.dt_next;
+  }
 }
 
 template <int NS, int LO, bool MIXED, bool CP>
@@ -296,17 +315,20 @@ using namespace streams;
 
 // Batched integration of nbatch independent reactors (rosenbrock_kernel per reactor, or its
 // mixed-precision variant when mixed != 0) from t = 0 to time_end[b]: constant volume when
-// pressure is null, else constant pressure pressure[b] with the state (Y, 0, 0, 0, h).  work
-// points to one unsigned of device memory used as the shared work counter (zeroed here, on
-// the stream); wg_per_cu <= 0 lets the occupancy calculator size the persistent grid.
+// pressure is null, else constant pressure pressure[b] with the state (Y, 0, 0, 0, h).  With
+// t_done / dt_next (both or neither) the call is resumable: each reactor starts at
+// t_done[b] with the step dt_next[b] (dttry[b] when not positive), takes at most maxsteps
+// steps, and returns the time reached and its next step there.  work points to one unsigned
+// of device memory used as the shared work counter (zeroed here, on the stream);
+// wg_per_cu <= 0 lets the occupancy calculator size the persistent grid.
 extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
                                      const double *time_end, const double *pressure, const double *mw,
                                      const double *rgas, const double *cp_coeff, int nreactions,
                                      const double *kin_pack, const double *kc_tab, int num_t_tab, double t_min_tab,
                                      double dt_tab, double r_univ, double tol_iter_nr, const double *cv_coeff,
                                      int nsetcv, const double *trange, int indx_cp_l, int indx_cp_r, int maxsteps,
-                                     int maxtry, double tol, int *nacc, int *nrej, unsigned *work, int wg_per_cu,
-                                     int mixed, void *stream) {
+                                     int maxtry, double tol, int *nacc, int *nrej, double *t_done, double *dt_next,
+                                     unsigned *work, int wg_per_cu, int mixed, void *stream) {
   if (nbatch < 0 || nreactions < 0 || maxsteps < 1 || maxtry < 1) return STREAMS_E_DIMS;
   if (n_s > KTB) return STREAMS_E_NSPECIES;
   if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;
@@ -314,7 +336,9 @@ extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tgu
   if (work == nullptr) return STREAMS_E_DIMS;
   hipStream_t st = as_stream(stream);
   if (pressure != nullptr && (rgas == nullptr || cp_coeff == nullptr)) return STREAMS_E_DIMS;
-  ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, work, nbatch, maxsteps, maxtry, mixed != 0, tol, tol_iter_nr, mw,
+  if ((t_done == nullptr) != (dt_next == nullptr)) return STREAMS_E_DIMS;
+  ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, t_done, dt_next, work, nbatch, maxsteps, maxtry, mixed != 0, tol,
+                tol_iter_nr, mw,
                 rgas, pressure, KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                 Thermo{cv_coeff, cp_coeff, trange, nsetcv, indx_cp_l, indx_cp_r}};
   zero_counter_kernel<<<1, 1, 0, st>>>(work);

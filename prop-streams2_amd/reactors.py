@@ -38,18 +38,22 @@ class ReactorBatchGPU:
         self.work = backend.zeros((2,), "i4")
 
     def integrate(self, w, tg, dttry, time_end, nacc=None, nrej=None, wg_per_cu: int = 0, stream=None,
-                  mixed: bool = False, pressure=None, work=None):
+                  mixed: bool = False, pressure=None, work=None, t_done=None, dt_next=None,
+                  maxsteps: Optional[int] = None):
         """w (nv, nb) float64, tg (nb) float64, dttry/time_end (nb) float64, all on the device;
         nacc/nrej optional int32 (nb).  Enqueued on `stream` (default: the backend's).
-        mixed=True: FP32-factored iteration matrix with FP64 residual refinement.
+        mixed=True: the species-block integrator (FP32 factor, FP64 refinement).
         pressure (nb) on the device: constant-pressure reactors with w = (Y, 0, 0, 0, h).
-        work: a private counter for launches that may overlap on other streams."""
+        work: a private counter for launches that may overlap on other streams.
+        t_done/dt_next (nb) float64 on the device: resumable call, at most `maxsteps` steps
+        per reactor from t_done with the step dt_next, both updated."""
         tb = self.tb
         nb = int(w.shape[1])
         self.be.call("reactor_batch", self.ns, nb, w, tg, dttry, time_end, pressure, self.t_mw, self.t_rgas,
                      self.t_cp, tb.nreactions, self.t_kin, self.t_kc, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV,
-                     TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.maxsteps,
-                     self.maxtry, self.tol, nacc, nrej, self.work if work is None else work, wg_per_cu, int(mixed),
+                     TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r,
+                     self.maxsteps if maxsteps is None else maxsteps, self.maxtry, self.tol, nacc, nrej, t_done,
+                     dt_next, self.work if work is None else work, wg_per_cu, int(mixed),
                      self.be.stream if stream is None else stream)
 
 
@@ -68,17 +72,29 @@ def reactor_inputs(tables: Tables, n: int, seed: int = 7, dt: float = 1e-6, dttr
     return w, tg, np.full(n, dttry), np.full(n, dt), info
 
 
+def mixture_r(tables: Tables, Y: np.ndarray) -> np.ndarray:
+    """R_mix of many states (n, N_S), summed species by species: the same bits for a state
+    whatever else is in the array (a BLAS product's blocking depends on the batch)."""
+    r = np.zeros(Y.shape[0])
+    for s in range(Y.shape[1]):
+        r = r + Y[:, s] * tables.rgas[s]
+    return r
+
+
 def temperature_from_e(tables: Tables, e: np.ndarray, Y: np.ndarray, t_guess: np.ndarray) -> np.ndarray:
     """Newton inversion of the mixture internal energy on the host for many reactor states
     (the integrator's own temperature output belongs to its last right-hand side)."""
     T = np.array(t_guess, dtype=np.float64)
-    r = Y @ tables.rgas
+    r = mixture_r(tables, Y)
+    act = np.arange(T.size)            # each state stops on its own convergence test
     for _ in range(50):
-        f = tables.e_from_T_many(T, Y) - e
-        dT = f / (tables.cp_from_T_many(T, Y) - r)
-        T = T - dT
-        if np.all(np.abs(dT) <= 1e-11 * T):
+        if act.size == 0:
             break
+        Ta, Ya = T[act], Y[act]
+        f = tables.e_from_T_many(Ta, Ya) - e[act]
+        dT = f / (tables.cp_from_T_many(Ta, Ya) - r[act])
+        T[act] = Ta - dT
+        act = act[~(np.abs(dT) <= 1e-11 * T[act])]
     return T
 
 

@@ -31,7 +31,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from .mechanism import Tables
-from .reactors import ROS_DTTRY, ReactorBatchGPU
+from .reactors import ROS_DTTRY, ReactorBatchGPU, mixture_r
 
 
 @dataclasses.dataclass
@@ -57,7 +57,7 @@ class Request:
 
 def enthalpy_many(tables: Tables, T: np.ndarray, Y: np.ndarray) -> np.ndarray:
     """Mixture enthalpy h = e + R T for many states."""
-    return tables.e_from_T_many(T, Y) + (Y @ tables.rgas) * T
+    return tables.e_from_T_many(T, Y) + mixture_r(tables, Y) * T
 
 
 def temperature_from_h(tables: Tables, h: np.ndarray, Y: np.ndarray, t_guess: np.ndarray) -> np.ndarray:
@@ -65,22 +65,25 @@ def temperature_from_h(tables: Tables, h: np.ndarray, Y: np.ndarray, t_guess: np
     back from the device: the integrator's own temperature output belongs to its last
     right-hand side evaluation)."""
     T = np.array(t_guess, dtype=np.float64)
-    r = Y @ tables.rgas
-    for _ in range(50):
-        f = tables.e_from_T_many(T, Y) + r * T - h
-        dT = f / tables.cp_from_T_many(T, Y)
-        T = T - dT
-        if np.all(np.abs(dT) <= 1e-11 * T):
+    r = mixture_r(tables, Y)
+    act = np.arange(T.size)            # states still iterating: each stops on its own test, so
+    for _ in range(50):                # a state's result does not depend on the others in the call
+        if act.size == 0:
             break
+        Ta, Ya = T[act], Y[act]
+        f = tables.e_from_T_many(Ta, Ya) + r[act] * Ta - h[act]
+        dT = f / tables.cp_from_T_many(Ta, Ya)
+        T[act] = Ta - dT
+        act = act[~(np.abs(dT) <= 1e-11 * T[act])]
     return T
 
 
 class Pool:
     """Device-resident slots of one mechanism."""
 
-    BYTES_PER_SLOT_EXTRA = 5 * 8 + 2 * 4   # tg, first step, slice, pressure, copy; counters
+    BYTES_PER_SLOT_EXTRA = 7 * 8 + 2 * 4   # tg, first step, slice, pressure, time done, next step, copy; counters
 
-    def __init__(self, backend, tables: Tables, capacity: int, slices: int, mixed: bool):
+    def __init__(self, backend, tables: Tables, capacity: int, slices: int, mixed: bool, budget: int = 0):
         import torch
         self.torch = torch
         self.be = backend
@@ -90,6 +93,7 @@ class Pool:
         self.cap = capacity
         self.slices = slices
         self.mixed = mixed
+        self.budget = budget                 # integrator steps per reactor and tick (0: whole slices)
         self.gpu = ReactorBatchGPU(backend, tables, tol=1e-8)
         self.gpu_fp64 = self.gpu
         self.stream = torch.cuda.Stream(device=backend.device)          # default priority
@@ -101,6 +105,8 @@ class Pool:
         self.first = torch.zeros(capacity, dtype=f8, device=dev)
         self.step = torch.zeros(capacity, dtype=f8, device=dev)
         self.pres = torch.zeros(capacity, dtype=f8, device=dev)
+        self.tdone = torch.zeros(capacity, dtype=f8, device=dev)      # time reached in the current slice
+        self.dtn = torch.zeros(capacity, dtype=f8, device=dev)        # next step size (0: start with first)
         self.nacc = torch.zeros(capacity, dtype=torch.int32, device=dev)
         # host bookkeeping
         self.free = list(range(capacity - 1, -1, -1))
@@ -129,6 +135,8 @@ class Pool:
             self.pres[idx] = torch.from_numpy(req.pressure.copy()).to(self.be.device)
             self.first[idx] = ROS_DTTRY
             self.step[idx] = req.horizon / self.slices
+            self.tdone[idx] = 0.0
+            self.dtn[idx] = 0.0
         req.ignition = np.full(req.n, np.nan)
         req._slots = slots
         req._remaining = req.n
@@ -142,7 +150,8 @@ class Pool:
         return sorted(self.owner)
 
     def launch(self, slots: List[int]):
-        """Advance the given slots by one slice (asynchronously on the pool's stream)."""
+        """Advance the given slots (asynchronously on the pool's stream): by one slice, or with
+        a step budget by at most `budget` integrator steps within their current slice."""
         torch = self.torch
         if not slots:
             return None
@@ -154,11 +163,19 @@ class Pool:
             step = self.step.index_select(0, idx).contiguous()
             pres = self.pres.index_select(0, idx).contiguous()
             nacc = torch.zeros(len(slots), dtype=torch.int32, device=self.be.device)
-            w0 = w.clone()
-            t0 = tg.clone()
-            self.gpu.integrate(w, tg, first, step, nacc, None, stream=self.stream.cuda_stream, mixed=self.mixed,
-                               pressure=pres, work=self.work)
-        return dict(idx=idx, slots=slots, w=w, tg=tg, w0=w0, t0=t0, first=first, step=step, pres=pres, nacc=nacc)
+            job = dict(idx=idx, slots=slots, w=w, tg=tg, w0=w.clone(), t0=tg.clone(), first=first, step=step,
+                       pres=pres, nacc=nacc)
+            if self.budget > 0:
+                job["td"] = self.tdone.index_select(0, idx).contiguous()
+                job["dn"] = self.dtn.index_select(0, idx).contiguous()
+                job["td0"], job["dn0"] = job["td"].clone(), job["dn"].clone()
+                self.gpu.integrate(w, tg, first, step, nacc, None, stream=self.stream.cuda_stream, mixed=self.mixed,
+                                   pressure=pres, work=self.work, t_done=job["td"], dt_next=job["dn"],
+                                   maxsteps=self.budget)
+            else:
+                self.gpu.integrate(w, tg, first, step, nacc, None, stream=self.stream.cuda_stream, mixed=self.mixed,
+                                   pressure=pres, work=self.work)
+        return job
 
     def finish(self, job, now: float, completed: List[Request]):
         """Collect one launch: fallback for failed reactors, write back, advance slices and
@@ -168,33 +185,58 @@ class Pool:
             return
         self.stream.synchronize()
         w = job["w"]
-        bad = (~torch.isfinite(w).all(dim=0)) | (job["nacc"] >= self.gpu.maxsteps)
+        budgeted = "td" in job
+        bad = ~torch.isfinite(w).all(dim=0)
+        if not budgeted:
+            bad = bad | (job["nacc"] >= self.gpu.maxsteps)
         nbad = int(bad.sum().item())
         if nbad and self.mixed:
             bi = torch.nonzero(bad).flatten()
+            sel = lambda a: a.index_select(0, bi).contiguous()           # noqa: E731
             wb = job["w0"].index_select(1, bi).contiguous()
-            tb_ = job["t0"].index_select(0, bi).contiguous()
+            tb_ = sel(job["t0"])
             with torch.cuda.stream(self.stream):
-                self.gpu_fp64.integrate(wb, tb_, job["first"].index_select(0, bi).contiguous(),
-                                        job["step"].index_select(0, bi).contiguous(), None, None,
-                                        stream=self.stream.cuda_stream, mixed=False,
-                                        pressure=job["pres"].index_select(0, bi).contiguous(), work=self.work)
+                if budgeted:
+                    td, dn = sel(job["td0"]), sel(job["dn0"])
+                    self.gpu_fp64.integrate(wb, tb_, sel(job["first"]), sel(job["step"]), None, None,
+                                            stream=self.stream.cuda_stream, mixed=False, pressure=sel(job["pres"]),
+                                            work=self.work, t_done=td, dt_next=dn, maxsteps=self.budget)
+                    job["td"][bi] = td
+                    job["dn"][bi] = dn
+                else:
+                    self.gpu_fp64.integrate(wb, tb_, sel(job["first"]), sel(job["step"]), None, None,
+                                            stream=self.stream.cuda_stream, mixed=False, pressure=sel(job["pres"]),
+                                            work=self.work)
                 w[:, bi] = wb
                 job["tg"][bi] = tb_
             self.stream.synchronize()
         idx = job["idx"]
         self.w[:, idx] = w
         self.tg[idx] = job["tg"]
+        if budgeted:
+            slice_done = (job["td"] >= job["step"])
+            # a finished slice starts the next one afresh (time 0, first step), as whole-slice
+            # ticks do; an unfinished one resumes where it stopped
+            self.tdone[idx] = torch.where(slice_done, torch.zeros_like(job["td"]), job["td"])
+            self.dtn[idx] = torch.where(slice_done, torch.zeros_like(job["dn"]), job["dn"])
+            done_h = slice_done.cpu().numpy()
+        else:
+            done_h = np.ones(len(job["slots"]), dtype=bool)
         wh = w.cpu().numpy()
         tgh = job["tg"].cpu().numpy()
         badh = bad.cpu().numpy()
         ys = np.maximum(wh[:self.ns], 0.0).T
         ys = ys / ys.sum(axis=1, keepdims=True)
-        temps = temperature_from_h(self.tb, wh[self.ns + 3], ys, tgh)
+        cols = np.nonzero(done_h)[0]
+        temps = np.full(len(job["slots"]), np.nan)
+        if cols.size:
+            temps[cols] = temperature_from_h(self.tb, wh[self.ns + 3, cols], ys[cols], tgh[cols])
         for c, s in enumerate(job["slots"]):
             req, k = self.owner[s]
             if badh[c]:
                 req.fallbacks += 1
+            if not done_h[c]:
+                continue
             self.slice_no[s] += 1
             tt = temps[c]
             thr = self.t0[s] + 400.0
@@ -220,13 +262,14 @@ class Pool:
 
 class ReactorServer:
     def __init__(self, backend, mechanisms: Dict[str, Tables], hbm_bytes: int = 1 << 30, slices: int = 20,
+                 budget: int = 0,
                  mixed: bool = True):
         self.be = backend
         self.pools = {}
         for name, tb in mechanisms.items():
             per = 8 * (tb.n_species + 4) + Pool.BYTES_PER_SLOT_EXTRA
             cap = max(1, int(hbm_bytes // len(mechanisms) // per))
-            self.pools[name] = Pool(backend, tb, cap, slices, mixed)
+            self.pools[name] = Pool(backend, tb, cap, slices, mixed, budget)
 
     def run(self, requests: List[Request], realtime: bool = True, max_seconds: float = 600.0):
         """Serve `requests` (sorted by arrival).  realtime=True admits a request only once

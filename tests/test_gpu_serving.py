@@ -25,8 +25,8 @@ def setup(pkg):
     return hip, mechs, reqs
 
 
-def _serve(pkg, hip, mechs, reqs, mixed=True):
-    srv = pkg.serving.ReactorServer(hip, mechs, hbm_bytes=1 << 24, slices=8, mixed=mixed)
+def _serve(pkg, hip, mechs, reqs, mixed=True, budget=0):
+    srv = pkg.serving.ReactorServer(hip, mechs, hbm_bytes=1 << 24, slices=8, mixed=mixed, budget=budget)
     stats = srv.run(copy.deepcopy(reqs), realtime=True, max_seconds=120)
     return stats
 
@@ -63,3 +63,19 @@ def test_mixed_precision_server_against_fp64_judge(pkg, setup):
             worst_ign = max(worst_ign, float(np.max(np.abs(ia[ok] - ib[ok]) / ib[ok])))
     print(f"[serving] mixed vs FP64: final T {worst_t:.2e}, ignition delay {worst_ign:.2e}", flush=True)
     assert worst_t <= 1e-6 and worst_ign <= 1e-6
+
+
+def test_step_budget_gives_the_same_results(pkg, setup):
+    """Ticks of at most 25 integrator steps per reactor (resumed from the time and step
+    reached) give the same states as whole slices per tick: resuming continues the step
+    sequence exactly."""
+    hip, mechs, reqs = setup
+    a = {r.rid: r for r in _serve(pkg, hip, mechs, reqs, budget=0)["completed"]}
+    run_b = _serve(pkg, hip, mechs, reqs, budget=25)
+    b = {r.rid: r for r in run_b["completed"]}
+    assert set(a) == set(b)
+    for rid in a:
+        np.testing.assert_array_equal(a[rid].t_final, b[rid].t_final)
+        np.testing.assert_array_equal(a[rid].y_final, b[rid].y_final)
+    assert run_b["ticks"] > 8
+
