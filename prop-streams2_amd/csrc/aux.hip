@@ -17,6 +17,9 @@ struct AuxTables {
   const int *nainsp;
 };
 
+// The mixture-rule divisions of the transport kernel use rcp_nr (reciprocal estimate + two
+// Newton steps): the transport coefficients then differ from the reference's in the last
+// bits only.
 // The thermodynamic state (eval_aux_cuf up to the mixture fraction, kernels_gpu.F90:6154-6251)
 // and the mixture transport (:6252-6300) run as two kernels over the same points: apart,
 // each holds far fewer registers than the pair would, for the price of re-reading Y, T, p
@@ -141,9 +144,9 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
     double klsp = l0 + (l1 - l0) * dtt;
     const double xs = yy[s] * mwmixt * tb.mwinv[s];
     k1 = k1 + xs * klsp;
-    k2 = k2 + xs / klsp;
+    k2 = k2 + xs * rcp_nr(klsp);
   }
-  const double k_cond = 0.5 * (k1 + 1.0 / k2);
+  const double k_cond = 0.5 * (k1 + rcp_nr(k2));
   // Wilke viscosity: sqrt(mu_m/mu_l) evaluated as sqrt(mu_m) * (1/sqrt(mu_l))
   double mus[NS], smu[NS];
 #pragma unroll
@@ -157,7 +160,7 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
   double mu = 0.0;
 #pragma unroll
   for (int l = 0; l < NS; ++l) {      // unrolled (no register indexing), one l at a time
-    const double isl = 1.0 / smu[l];
+    const double isl = rcp_nr(smu[l]);
     double den = 0.0;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
@@ -165,7 +168,7 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
       double phi = c_inv[l + NS * m] * (t1 * t1);
       den = den + yy[m] * phi * tb.mwinv[m];
     }
-    mu = mu + mus[l] * yy[l] * tb.mwinv[l] / den;
+    mu = mu + mus[l] * yy[l] * tb.mwinv[l] * rcp_nr(den);
     __builtin_amdgcn_sched_barrier(0);
   }
   mu = sqrt(8.0) * mu;
@@ -190,7 +193,7 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
         const double *dp = tb.diffbin + (itt - 1) + nt * (l + NS * m);
         double d0 = dp[0], d1 = dp[1];
         double dij = d0 + (d1 - d0) * dtt;
-        dden = dden + xs[m] / dij;
+        dden = dden + xs[m] * rcp_nr(dij);
       }
     }
     w_aux[pt + ps * (X::J_D_START + l)] = (dden > 1.0e-15) ? (1.0 - yy[l]) / dden * fac : 0.0;

@@ -35,6 +35,17 @@ struct Box {
 // are recomputed rather than kept live (in VGPRs) across the code in between.
 template <class T> __device__ inline void opaque(T &x) { asm volatile("" : "+v"(x)); }
 
+// 1/d from the hardware reciprocal estimate refined by two Newton steps (about one ulp,
+// not correctly rounded): a fraction of the instructions of an IEEE division, for the
+// transport sums that divide by table values dozens of times per point.
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
 // index of variables (index_define.h), 1-based as in the reference
 template <int NS> struct Idx {
   static constexpr int I_U = NS + 1, I_V = NS + 2, I_W = NS + 3, I_E = NS + 4;
