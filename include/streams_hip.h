@@ -289,8 +289,8 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
  * dt_next[b] (dttry[b] if not positive), takes at most maxsteps steps and returns the time
  * reached (time_end[b] when done) and the step it would take next.  Kinetics come from the device copy of
  * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
- * counter (zeroed by the call on the stream); wg_per_cu <= 0 sizes the persistent grid to
- * the resident capacity.  mixed = 0 is the FP64 integrator; mixed = 1 solves on the species
+ * counter (zeroed by the call on the stream); wg_per_cu workgroups of 256 lanes per CU form
+ * the persistent grid (4 when wg_per_cu <= 0; never more than the batch needs).  mixed = 0 is the FP64 integrator; mixed = 1 solves on the species
  * block (the momentum / energy rows of a reactor's RHS are zero), with a one-sided species
  * Jacobian factored in FP32 and every stage solve refined once against the FP64 residual
  * (same stages, error norm and step control).  No

@@ -280,21 +280,28 @@ __global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
 
 __global__ void zero_counter_kernel(unsigned *c) { *c = 0u; }
 
-// resident workgroups of reactor_kernel<NS, LO> on this device (persistent grid size)
-template <int NS, int LO, bool MIXED, bool CP>
-int reactor_grid(int wg_per_cu) {
+// Persistent grid: wg_per_cu workgroups per CU (default 4, at least the resident capacity
+// of these register-heavy kernels), never more than the batch needs.  Any size is correct:
+// lanes pull reactors from the shared counter until it runs out, so workgroups queued
+// behind the resident ones start after it has run out and exit at once.  The occupancy
+// API is not used: its query has returned error 999 after many other kernels had been
+// launched in the process.
+inline int reactor_grid(int wg_per_cu, long nbatch) {
   int dev = 0, ncu = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reactor_kernel<NS, LO, MIXED, CP>, 256, 0) != hipSuccess) occ = 1;
-  if (wg_per_cu > 0 && wg_per_cu < occ) occ = wg_per_cu;
-  return ncu * (occ > 0 ? occ : 1);
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      ncu <= 0) {
+    (void)hipGetLastError();
+    ncu = 256;
+  }
+  const long per = wg_per_cu > 0 ? wg_per_cu : 4;
+  const long need = (nbatch + 255) / 256;
+  const long g = ncu * per;
+  return (int)(g < need ? g : need);
 }
 
 template <int NS, int LO, bool MIXED, bool CP>
 int reactor_launch_t(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
-  const int g = reactor_grid<NS, LO, MIXED, CP>(wg_per_cu);
+  const int g = reactor_grid(wg_per_cu, A.nbatch);
   if (g <= 0) return STREAMS_E_DIMS;
   reactor_kernel<NS, LO, MIXED, CP><<<g, 256, 0, st>>>(A);
   return hip_status();
@@ -320,7 +327,7 @@ using namespace streams;
 // t_done[b] with the step dt_next[b] (dttry[b] when not positive), takes at most maxsteps
 // steps, and returns the time reached and its next step there.  work points to one unsigned
 // of device memory used as the shared work counter (zeroed here, on the stream);
-// wg_per_cu <= 0 lets the occupancy calculator size the persistent grid.
+// wg_per_cu <= 0 gives 4 workgroups per CU.
 extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,
                                      const double *time_end, const double *pressure, const double *mw,
                                      const double *rgas, const double *cp_coeff, int nreactions,
