@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--reactors", type=int, default=0,
                     help="also time the batched chemistry on this many reactors (e.g. 2097152)")
