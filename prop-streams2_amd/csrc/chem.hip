@@ -3,7 +3,7 @@
 // compute_chemistry: one thread per interior cell, reaction loop over the tables in
 // their Fortran layout (arr_a(nr,2), tb_eff(nr,N_S), kc_tab(num_t_tab+1,nr), ...).
 // rosenbrock: one thread per cell restating rosenbrock_kernel with its central-difference
-// Jacobian and Crout LU; it is the FP64 judge for the batched integrators of reactors.hip.
+// Jacobian and Crout LU; it is the FP64 yardstick for the batched integrators of reactors.hip.
 #include "chem_common.h"
 
 namespace streams {

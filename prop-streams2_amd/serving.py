@@ -16,7 +16,7 @@ request gives the same result alone and in any mix (each reactor integrates the 
 slices of its own horizon).  The mixed-precision integrator is the default; a reactor whose
 slice fails (non-finite state or the step limit) is recomputed from the slice's initial
 state with the FP64 integrator (fallback), and a sample of requests is shadowed by the
-FP64 integrator to report the largest difference (judge).
+FP64 integrator to report the largest difference.
 
 Ignition delay: the first slice end with T > T0 + 400 K, linearly interpolated between slice
 ends (the resolution is horizon / slices).

@@ -1,6 +1,6 @@
 """GPU parity: each HIP entry point against the CPU restatement on identical seeded inputs.
 
-Tolerance (north_star): RHS agreement within 1e-12 relative in the max-norm of each
+Tolerance: RHS agreement within 1e-12 relative in the max-norm of each
 conserved variable; species and temperature after N steps within 1e-9 relative.
 Auxiliary variables are held to the same 1e-12.  Copies (halo exchange) are bit-exact.
 Shapes are deliberately non-cubic so that a transposed stride or a swapped extent fails.

@@ -1,6 +1,6 @@
 """The continuous-batching reactor server on the GPU: a request's result is the same alone
 and in a mix of requests of both mechanisms arriving over time, and the mixed-precision
-results agree with the FP64 integrator (the judge) within 1e-6 in final T and ignition
+results agree with the FP64 integrator within 1e-6 in final T and ignition
 delay."""
 import copy
 
@@ -48,7 +48,7 @@ def test_request_same_alone_and_in_the_mix(pkg, setup):
     assert mix["latency_p99_s"] >= mix["latency_p50_s"] > 0
 
 
-def test_mixed_precision_server_against_fp64_judge(pkg, setup):
+def test_mixed_precision_server_against_fp64(pkg, setup):
     hip, mechs, reqs = setup
     a = {r.rid: r for r in _serve(pkg, hip, mechs, reqs, mixed=True)["completed"]}
     b = {r.rid: r for r in _serve(pkg, hip, mechs, reqs, mixed=False)["completed"]}
