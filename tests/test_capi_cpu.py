@@ -72,3 +72,14 @@ def test_kinetics_pack_restates_the_dense_tables(pkg):
             assert ints[r, 6] == int(tb.falloff_coeffs[r, 3] != -3.14)
         np.testing.assert_array_equal(r_co, tb.r_coeffs)
         np.testing.assert_array_equal(p_co, tb.p_coeffs)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py drives one GPU; --gpus 2 exits with status 2 and says so, before touching a GPU."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "--gpus 2 is not supported" in r.stderr
+    assert r.stdout == ""
