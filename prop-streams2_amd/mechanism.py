@@ -42,6 +42,7 @@ import yaml
 
 R_UNIV = 8314.46261815324      # J/(kmol K), src/parameters.F90:54
 ONE_ATM = 101325.0
+NAME_LEN = 30                  # character(len=30) species_names, multideal.F90:213
 AVOGADRO = 6.02214076e26       # 1/kmol
 BOLTZMANN = 1.380649e-23       # J/K
 EPS0 = 8.8541878128e-12        # F/m
@@ -475,6 +476,13 @@ class Mechanism:
         h298 = np.array([self.h_RT_s_R(k, np.array([298.15]))[0][0] * R_UNIV * 298.15
                          for k in range(self.n_species)])
         out["h298"] = h298
+        # transport data in the units of fluid_prop.bin (tools/preproc_multi/preproc.py:136-140):
+        # diameter m, well depth / k_B K, dipole C m, polarizability m^3
+        out["sigma"] = np.array([s.diameter for s in self.species]) * 1e-10
+        out["eps_k"] = np.array([s.well_depth for s in self.species])
+        out["dipole"] = np.array([s.dipole for s in self.species]) * DEBYE
+        out["polariz"] = np.array([s.polarizability for s in self.species]) * 1e-30
+        out["geometry"] = np.array([s.geometry for s in self.species], dtype=np.int32)
         return Tables(**out)
 
 
@@ -529,6 +537,13 @@ class Tables:
     r_coeffs: Optional[np.ndarray] = None
     p_coeffs: Optional[np.ndarray] = None
     kc_tab: Optional[np.ndarray] = None
+    # species transport data (fluid_prop.bin records 6-10; used by the reference only for
+    # its Soret collision integrals)
+    sigma: Optional[np.ndarray] = None
+    eps_k: Optional[np.ndarray] = None
+    dipole: Optional[np.ndarray] = None
+    polariz: Optional[np.ndarray] = None
+    geometry: Optional[np.ndarray] = None
 
     @property
     def n_species(self) -> int:
@@ -624,11 +639,12 @@ class Tables:
         return r
 
     # ------------------------------------------------------- fluid_prop.bin I/O
-    def write_fluid_prop(self, path: str, sigma=None, eps_k=None, dipole=None, polariz=None, geometry=None):
+    def write_fluid_prop(self, path: str):
         """Write the Fortran sequential-unformatted file read by set_fluid_prop
-        (multideal.F90:2763-2926, written by tools/preproc_multi/preproc.py:202-246).
+        (multideal.F90:2763-2926, written by tools/preproc_multi/preproc.py:196-246).
 
-        Records carry 4-byte length markers.  The transport tables are stored at
+        Records carry 4-byte length markers.  Species names are character(len=30)
+        (multideal.F90:213, preproc.py:197-198).  The transport tables are stored at
         1 atm, as the reader rescales them by 101325/p0.
         """
         ns = self.n_species
@@ -640,13 +656,13 @@ class Tables:
             recs.append(struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload)))
 
         rec(np.array([ns, self.nreactions], dtype=np.int32))
-        rec(b"".join(n.ljust(40)[:40].encode() for n in self.names))
+        rec(b"".join(n.ljust(NAME_LEN)[:NAME_LEN].encode() for n in self.names))
         rec(self.mw.astype(np.float64))
         rec(self.rgas.astype(np.float64))
         rec(self.h298.astype(np.float64))
-        for arr in (sigma, eps_k, dipole, polariz):
+        for arr in (self.sigma, self.eps_k, self.dipole, self.polariz):
             rec((z if arr is None else np.asarray(arr)).astype(np.float64))
-        rec((np.zeros(ns, dtype=np.int32) if geometry is None else np.asarray(geometry, dtype=np.int32)))
+        rec((np.zeros(ns, dtype=np.int32) if self.geometry is None else np.asarray(self.geometry, dtype=np.int32)))
         ncoef = self.indx_cp_r + 3 - self.indx_cp_l
         rec(np.array([ncoef, self.nsetcv], dtype=np.int32))
         rec(np.asfortranarray(self.trange))
@@ -678,13 +694,14 @@ def read_fluid_prop(path: str, p0: float) -> Tables:
     ns, nr = np.frombuffer(next(it), dtype=np.int32)
     ns, nr = int(ns), int(nr)
     nraw = next(it)
-    width = len(nraw) // ns
-    names = [nraw[i * width:(i + 1) * width].decode().strip() for i in range(ns)]
+    if len(nraw) != NAME_LEN * ns:
+        raise ValueError(f"species-name record of {len(nraw)} bytes; set_fluid_prop reads {ns} x {NAME_LEN}")
+    names = [nraw[i * NAME_LEN:(i + 1) * NAME_LEN].decode().strip() for i in range(ns)]
     mw = np.frombuffer(next(it), dtype=np.float64).copy()
     rgas = np.frombuffer(next(it), dtype=np.float64).copy()
     h298 = np.frombuffer(next(it), dtype=np.float64).copy()
-    for _ in range(5):
-        next(it)
+    tr = {k: np.frombuffer(next(it), dtype=np.float64).copy() for k in ("sigma", "eps_k", "dipole", "polariz")}
+    tr["geometry"] = np.frombuffer(next(it), dtype=np.int32).copy()
     ncoef, nsetcv = (int(v) for v in np.frombuffer(next(it), dtype=np.int32))
     lo, hi = {7: (0, 4), 9: (-2, 4)}[ncoef]
     trange = np.frombuffer(next(it), dtype=np.float64).reshape((ns, nsetcv + 1), order="F").copy(order="F")
@@ -714,4 +731,4 @@ def read_fluid_prop(path: str, p0: float) -> Tables:
     return Tables(names=names, indx_cp_l=lo, indx_cp_r=hi, nsetcv=nsetcv, trange=trange, cp_coeff=cp,
                   cv_coeff=cv, rgas=rgas, mw=mw, t_min_tab=float(tmin), t_max_tab=float(tmax),
                   dt_tab=float(dtt), num_t_tab=num, p0=p0, h298=h298, nreactions=nr, visc_species=visc,
-                  lambda_species=lam, diffbin_species=dif, **kw)
+                  lambda_species=lam, diffbin_species=dif, **tr, **kw)

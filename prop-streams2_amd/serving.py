@@ -8,7 +8,7 @@ tick, with one persistent launch of the batched integrator per pool and tick.  T
 run on their own streams, so the two mechanisms' launches overlap on the device.  A request
 that arrives while others are in flight is admitted at the next tick as soon as its
 reactors fit in the pool's free device memory (admission by HBM footprint, first come
-first served), and its slots are freed when it completes, so the batch composition changes
+first served within each pool), and its slots are freed when it completes, so the batch composition changes
 from tick to tick (continuous batching, not a queue of whole batches).
 
 Every reactor's arithmetic is independent of which other reactors share its launch: a
@@ -114,6 +114,9 @@ class Pool:
         self.slice_no = np.zeros(capacity, dtype=np.int64)
         self.t0 = np.zeros(capacity)
         self.hist_prev = np.zeros(capacity)
+        # (request id, reactor index) pairs whose mixed-precision result is treated as failed
+        # on every tick, so that they take the FP64 fallback (a test hook)
+        self.force_fallback = set()
 
     def bytes_per_reactor(self) -> int:
         return 8 * self.nv + self.BYTES_PER_SLOT_EXTRA
@@ -155,6 +158,8 @@ class Pool:
         torch = self.torch
         if not slots:
             return None
+        # the slot arrays were last written on the caller's stream (finish)
+        self.stream.wait_stream(torch.cuda.current_stream(self.be.device))
         with torch.cuda.stream(self.stream):
             idx = torch.tensor(slots, dtype=torch.int64, device=self.be.device)
             w = self.w.index_select(1, idx).contiguous()
@@ -189,26 +194,35 @@ class Pool:
         bad = ~torch.isfinite(w).all(dim=0)
         if not budgeted:
             bad = bad | (job["nacc"] >= self.gpu.maxsteps)
+        if self.force_fallback:
+            forced = [c for c, s in enumerate(job["slots"])
+                      if (self.owner[s][0].rid, self.owner[s][1]) in self.force_fallback]
+            if forced:
+                bad[torch.tensor(forced, dtype=torch.int64, device=bad.device)] = True
         nbad = int(bad.sum().item())
         if nbad and self.mixed:
             bi = torch.nonzero(bad).flatten()
             sel = lambda a: a.index_select(0, bi).contiguous()           # noqa: E731
-            wb = job["w0"].index_select(1, bi).contiguous()
-            tb_ = sel(job["t0"])
+            # gathers and the fallback launch on the pool's stream, after the caller's
+            # stream has produced `bad` / `bi`
+            self.stream.wait_stream(torch.cuda.current_stream(self.be.device))
             with torch.cuda.stream(self.stream):
+                wb = job["w0"].index_select(1, bi).contiguous()
+                tb_ = sel(job["t0"])
                 if budgeted:
                     td, dn = sel(job["td0"]), sel(job["dn0"])
                     self.gpu_fp64.integrate(wb, tb_, sel(job["first"]), sel(job["step"]), None, None,
                                             stream=self.stream.cuda_stream, mixed=False, pressure=sel(job["pres"]),
                                             work=self.work, t_done=td, dt_next=dn, maxsteps=self.budget)
-                    job["td"][bi] = td
-                    job["dn"][bi] = dn
                 else:
                     self.gpu_fp64.integrate(wb, tb_, sel(job["first"]), sel(job["step"]), None, None,
                                             stream=self.stream.cuda_stream, mixed=False, pressure=sel(job["pres"]),
                                             work=self.work)
                 w[:, bi] = wb
                 job["tg"][bi] = tb_
+                if budgeted:
+                    job["td"][bi] = td
+                    job["dn"][bi] = dn
             self.stream.synchronize()
         idx = job["idx"]
         self.w[:, idx] = w
@@ -285,14 +299,7 @@ class ReactorServer:
                 break
             while pending and (not realtime or pending[0].arrival <= now):
                 waiting.append(pending.pop(0))
-            still = []
-            for r in waiting:                     # first come, first served while it fits
-                pool = self.pools[r.mech]
-                if not still and pool.fits(r):
-                    pool.admit(r)
-                else:
-                    still.append(r)
-            waiting = still
+            waiting = admit_fifo_per_pool(waiting, self.pools)
             jobs = {name: p.launch(p.active()) for name, p in self.pools.items()}
             if all(j is None for j in jobs.values()):
                 if pending:
@@ -308,6 +315,22 @@ class ReactorServer:
                 "reactor_solves_per_s": n_reactors / wall if wall > 0 else 0.0,
                 "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
                 "fallback_reactors": int(sum(r.fallbacks for r in completed)), "completed": completed}
+
+
+def admit_fifo_per_pool(waiting: List[Request], pools) -> List[Request]:
+    """Admit waiting requests first come, first served within each mechanism's pool: a
+    request that does not fit its pool blocks the later requests of that pool only, never
+    those of another pool.  Returns the requests still waiting, in arrival order."""
+    blocked = set()
+    still = []
+    for r in waiting:
+        pool = pools[r.mech]
+        if r.mech not in blocked and pool.fits(r):
+            pool.admit(r)
+        else:
+            blocked.add(r.mech)
+            still.append(r)
+    return still
 
 
 def poisson_requests(mechanisms: Dict[str, Tables], rate: float, duration: float, seed: int = 2024,

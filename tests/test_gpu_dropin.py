@@ -11,16 +11,34 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 EXE = ROOT / "prop-streams2_amd" / "lib" / "dropin_check"
 
 
-def test_generated_module_matches_header():
-    """The committed module is what the generator makes from the current header."""
+def _generator():
     import importlib.util
     spec = importlib.util.spec_from_file_location("gen", ROOT / "prop-streams2_amd/fortran/gen_interfaces.py")
     gen = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(gen)
-    names = {c for c, _ in gen.parse()}
+    return gen
+
+
+def test_generated_module_matches_header():
+    """The committed module is, byte for byte, what the generator makes from the current
+    header: a stale interface would pass the wrong arguments to an entry point."""
+    gen = _generator()
     text = (ROOT / "prop-streams2_amd/fortran/streams_kernels_hip.F90").read_text()
-    for n in names:
+    assert text == gen.render()
+    for n in {c for c, _ in gen.parse()}:
         assert f'bind(C, name="{n}")' in text, n
+
+
+def test_module_interfaces_have_header_arity():
+    """Every bind(C) interface declares exactly the header's argument list, in order."""
+    import re
+    gen = _generator()
+    text = (ROOT / "prop-streams2_amd/fortran/streams_kernels_hip.F90").read_text()
+    for cname, args in gen.parse():
+        m = re.search(r"function " + cname + r"\((.*?)\)\s*&\s*\n\s*bind\(C", text, flags=re.S)
+        assert m, cname
+        names = [a.strip() for a in m.group(1).replace("&", " ").split(",")]
+        assert names == [a for a, _ in args], cname
 
 
 @pytest.mark.gpu

@@ -208,16 +208,18 @@ def test_mixed_precision_matches_fp64_ignition(pkg, hip):
 
 def test_grid_size_does_not_change_results(pkg, hip):
     """The persistent grid only decides which lane takes which reactor from the shared
-    counter: 1 and 3 workgroups per CU (256 and up to 768 workgroups for 70,000 reactors)
-    give bit-identical states and step counts."""
+    counter.  600,000 reactors need 2,344 workgroups of 256 lanes, so 1 workgroup per CU
+    (256 workgroups, all resident) and 8 per CU (2,048 workgroups, more than the GPU holds
+    at once for this register-heavy kernel: the queued ones start after the counter has run
+    out and exit at once) give bit-identical states and step counts."""
     tables = pkg.cases.h2_air_mechanism().tables()
-    n = 70000
+    n = 600000
     w, tg, _ = pkg.cases.reactor_batch(tables, n, seed=5)
     G = pkg.reactors.ReactorBatchGPU(hip, tables, tol=1e-8)
     horizon = hip.asarray(np.full(n, 1e-7))
     first = hip.asarray(np.full(n, pkg.reactors.ROS_DTTRY))
     out = []
-    for per in (1, 3):
+    for per in (1, 8):
         print(f"[reactors] grid test, wg_per_cu={per}, n={n}", flush=True)
         wd, td = hip.asarray(w), hip.asarray(tg)
         na, nr = hip.zeros((n,), "i4"), hip.zeros((n,), "i4")

@@ -79,3 +79,20 @@ def test_step_budget_gives_the_same_results(pkg, setup):
         np.testing.assert_array_equal(a[rid].y_final, b[rid].y_final)
     assert run_b["ticks"] > 8
 
+
+def test_fallback_equals_fp64_run(pkg, setup):
+    """A reactor whose mixed-precision result is treated as failed on every tick takes the
+    FP64 fallback from the tick's initial state; its final state then equals, bit for bit,
+    an FP64-only server's result for the same request."""
+    hip, mechs, reqs = setup
+    probe = reqs[1]
+    forced = {(probe.rid, k) for k in range(0, probe.n, 2)}
+    srv = pkg.serving.ReactorServer(hip, mechs, hbm_bytes=1 << 24, slices=8, mixed=True)
+    for p in srv.pools.values():
+        p.force_fallback = forced
+    got = srv.run(copy.deepcopy([probe]), realtime=False, max_seconds=120)["completed"][0]
+    ref = _serve(pkg, hip, mechs, [probe], mixed=False)["completed"][0]
+    assert got.fallbacks >= len(forced) * 8      # every slice of every forced reactor
+    ks = sorted(k for _, k in forced)
+    np.testing.assert_array_equal(got.t_final[ks], ref.t_final[ks])
+    np.testing.assert_array_equal(got.y_final[ks], ref.y_final[ks])
