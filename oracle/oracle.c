@@ -1399,11 +1399,74 @@ typedef struct {
     const int *reac_ty, *isrev;
     double t_min_tab, dt_tab, r_univ, tol_iter_nr;
     const double *cv_coeff, *trange;
+    /* constant-pressure reactor (pres > 0): state (Y_1..Y_NS, 0, 0, 0, h) */
+    const double *cp_coeff, *rgas;
+    double pres;
 } chem_t;
+
+static void ros_rhs_cv(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs);
+static void ros_rhs_cp(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs);
+
+/* The chemistry right-hand side of one reactor: the reference's constant-volume cell, or
+ * the constant-pressure reactor when C->pres > 0. */
+static void ros_rhs(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs)
+{
+    if (C->pres > 0.0) ros_rhs_cp(C, wv, yy, tguess, rhs);
+    else ros_rhs_cv(C, wv, yy, tguess, rhs);
+}
+
+/* Constant-pressure adiabatic reactor (no counterpart in the reference, whose reactor is a
+ * constant-volume cell).  State wv = (Y_1..Y_NS, 0, 0, 0, h), h the mixture enthalpy per
+ * unit mass, constant in time.  Written in rosenbrock_rhs's arithmetic forms
+ * (kernels_gpu.F90:9249-9339): species clipped at 0 and renormalised, T from h by the
+ * Newton iteration of get_mixture_temperature_from_h_roe_dev (:6810-6900, the e-inversion
+ * on the c_p tables), rho = p / (R_mix T), concentrations rho Y_s / W_s, the same rates of
+ * progress, and dY_s/dt = sum_r ((p - r) q_r) W_s / rho. */
+static void ros_rhs_cp(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs)
+{
+    int ns = C->ns, nv = C->nv, nreactions = C->nreactions, num_t_tab = C->num_t_tab;
+    int nsetcv = C->nsetcv, indx_cp_l = C->indx_cp_l, indx_cp_r = C->indx_cp_r;
+    double conc[NSMAX];
+    for (int lsp = 1; lsp <= ns; ++lsp) wv[lsp - 1] = wv[lsp - 1] > 0.0 ? wv[lsp - 1] : 0.0;
+    double ysum = wv[0];
+    for (int lsp = 2; lsp <= ns; ++lsp) ysum = ysum + wv[lsp - 1];
+    double yi = 1.0 / ysum;
+    double rmix = 0.0;
+    for (int lsp = 1; lsp <= ns; ++lsp) {
+        yy[lsp - 1] = wv[lsp - 1] * yi;
+        rmix = rmix + C->rgas[lsp - 1] * yy[lsp - 1];
+    }
+    double tt = temperature_from_e(ns, wv[I_E - 1], *tguess, C->cp_coeff, nsetcv, C->trange, indx_cp_l, indx_cp_r,
+                                   C->tol_iter_nr, yy);
+    *tguess = tt;
+    double rho = C->pres / (rmix * tt);
+    double ri = 1.0 / rho;
+    for (int lsp = 1; lsp <= ns; ++lsp) conc[lsp - 1] = rho * yy[lsp - 1] / C->mw[lsp - 1];
+    double Rtti = 1.0 / (C->r_univ * tt);
+    int itt = (int)((tt - C->t_min_tab) / C->dt_tab) + 1;
+    if (itt < 1) itt = 1;
+    if (itt > num_t_tab) itt = num_t_tab;
+    double ttleft = (itt - 1) * C->dt_tab + C->t_min_tab;
+    double dtt = (tt - ttleft) / C->dt_tab;
+    for (int m = 0; m < nv; ++m) rhs[m] = 0.0;
+    for (int lr = 1; lr <= nreactions; ++lr) {
+        double qlr = progress_rate(nreactions, lr, ns, tt, Rtti, itt, dtt, num_t_tab, conc, C->arr_a, C->arr_b,
+                                   C->arr_ea, C->tb_eff, C->falloff_coeffs, C->reac_ty, C->isrev, C->r_coeffs,
+                                   C->p_coeffs, C->kc_tab);
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            double rr = RX(C->r_coeffs, lr, lsp), pp = RX(C->p_coeffs, lr, lsp);
+            if (rr != 0.0 || pp != 0.0) {
+                double wdtkj = (pp - rr) * qlr;
+                rhs[lsp - 1] = rhs[lsp - 1] + wdtkj * C->mw[lsp - 1];
+            }
+        }
+    }
+    for (int lsp = 1; lsp <= ns; ++lsp) rhs[lsp - 1] = rhs[lsp - 1] * ri;
+}
 
 /* rosenbrock_rhs (kernels_gpu.F90:9195-9381) on the cell state wv[nv]; clips species in
  * wv, updates yy and the temperature guess *tguess */
-static void ros_rhs(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs)
+static void ros_rhs_cv(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs)
 {
     int ns = C->ns, nv = C->nv, nreactions = C->nreactions, num_t_tab = C->num_t_tab;
     int nsetcv = C->nsetcv, indx_cp_l = C->indx_cp_l, indx_cp_r = C->indx_cp_r;
@@ -1646,6 +1709,17 @@ int orc_rosenbrock(int ns, int nv, int nv_aux, int nx, int ny, int nz, int ng,
 
 /* Batched interface over a flat list of reactors (state w[nv][nbatch] SoA, T guess
  * tg[nbatch]) used by the batched-chemistry tests and the CPU baseline. */
+int orc_reactor_batch(int ns, int nbatch, double *w, double *tg, const double *mw, int nreactions,
+                      const double *arr_a, const double *arr_b, const double *arr_ea,
+                      const double *tb_eff, const double *falloff_coeffs, const int *reac_ty,
+                      const int *isrev, const double *r_coeffs, const double *p_coeffs,
+                      const double *kc_tab, int num_t_tab, double t_min_tab, double dt_tab,
+                      double r_univ, double tol_iter_nr, const double *cv_coeff, int nsetcv,
+                      const double *trange, int indx_cp_l, int indx_cp_r, const double *dttry,
+                      const double *time_end, int maxsteps, int maxtry, double tol,
+                      int *nacc, int *nrej, const double *pressure, const double *rgas,
+                      const double *cp_coeff);
+
 int orc_rosenbrock_batch(int ns, int nbatch, double *w, double *tg, const double *mw, int nreactions,
                          const double *arr_a, const double *arr_b, const double *arr_ea,
                          const double *tb_eff, const double *falloff_coeffs, const int *reac_ty,
@@ -1656,12 +1730,32 @@ int orc_rosenbrock_batch(int ns, int nbatch, double *w, double *tg, const double
                          const double *time_end, int maxsteps, int maxtry, double tol,
                          int *nacc, int *nrej)
 {
+    return orc_reactor_batch(ns, nbatch, w, tg, mw, nreactions, arr_a, arr_b, arr_ea, tb_eff, falloff_coeffs,
+                             reac_ty, isrev, r_coeffs, p_coeffs, kc_tab, num_t_tab, t_min_tab, dt_tab, r_univ,
+                             tol_iter_nr, cv_coeff, nsetcv, trange, indx_cp_l, indx_cp_r, dttry, time_end, maxsteps,
+                             maxtry, tol, nacc, nrej, NULL, NULL, NULL);
+}
+
+/* The same over constant-pressure reactors when pressure != NULL (pressure[b] in Pa, state
+ * (Y, 0, 0, 0, h), rgas and cp_coeff required), as streams_reactor_batch takes them. */
+int orc_reactor_batch(int ns, int nbatch, double *w, double *tg, const double *mw, int nreactions,
+                      const double *arr_a, const double *arr_b, const double *arr_ea,
+                      const double *tb_eff, const double *falloff_coeffs, const int *reac_ty,
+                      const int *isrev, const double *r_coeffs, const double *p_coeffs,
+                      const double *kc_tab, int num_t_tab, double t_min_tab, double dt_tab,
+                      double r_univ, double tol_iter_nr, const double *cv_coeff, int nsetcv,
+                      const double *trange, int indx_cp_l, int indx_cp_r, const double *dttry,
+                      const double *time_end, int maxsteps, int maxtry, double tol,
+                      int *nacc, int *nrej, const double *pressure, const double *rgas,
+                      const double *cp_coeff)
+{
     int nv = ns + 4;
-    chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
-                falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
-                tol_iter_nr, cv_coeff, trange};
+    if (pressure != NULL && (rgas == NULL || cp_coeff == NULL)) return -2;
     OMP_FOR
     for (int b = 0; b < nbatch; ++b) {
+        chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
+                    falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
+                    tol_iter_nr, cv_coeff, trange, cp_coeff, rgas, pressure ? pressure[b] : 0.0};
         double wv[NVMAX];
         for (int m = 0; m < nv; ++m) wv[m] = w[(size_t)m * nbatch + b];
         int na = 0, nr = 0;

@@ -1,10 +1,12 @@
-"""0-D constant-volume reactors on the oracle (test infrastructure).
+"""0-D reactors on the oracle (test infrastructure).
 
 A reactor is one cell of conserved variables w = (rho*Y_1..rho*Y_NS, rho*u, rho*v, rho*w,
 rho*E) integrated by the oracle's Rosenbrock restatement (orc_rosenbrock_batch, which
 follows rosenbrock_kernel, kernels_gpu.F90:8928-9193).  With no transport the reference's
 Strang-split step (advance_solution, multideal_gpu.F90:1881-1903) is two rosenbrock calls of
-dt/2 each starting with dttry = dt/2, which is what `advance` reproduces.
+dt/2 each starting with dttry = dt/2, which is what `advance` reproduces.  A
+constant-pressure reactor (no counterpart in the reference) has the state (Y_1..Y_NS, 0, 0,
+0, h) and integrates at a given pressure (orc_reactor_batch).
 """
 import ctypes
 
@@ -31,11 +33,12 @@ class ReactorBatch:
                       fortran(t.falloff_coeffs), fortran(t.reac_ty, np.int32), fortran(t.isRev, np.int32),
                       fortran(t.r_coeffs), fortran(t.p_coeffs), fortran(t.kc_tab), fortran(t.cv_coeff),
                       fortran(t.trange)]
-        fn = self.lib.orc_rosenbrock_batch
+        self._keep += [fortran(t.rgas), fortran(t.cp_coeff)]
+        fn = self.lib.orc_reactor_batch
         P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         fn.restype = I
         fn.argtypes = [I, I, P, P, P, I, P, P, P, P, P, P, P, P, P, P, I, D, D, D, D, P, I, P, I, I, P, P, I, I, D,
-                       P, P]
+                       P, P, P, P, P]
         self.fn = fn
 
     def state(self, temperature, pressure, y):
@@ -54,8 +57,21 @@ class ReactorBatch:
             w[self.ns + 3, b] = rho * self.tb.e_from_T(temperature[b], y[:, b])
         return w, temperature
 
-    def integrate(self, w, tg, dttry, time_end):
+    def state_cp(self, temperature, y):
+        """Constant-pressure reactor states (Y, 0, 0, 0, h) at (T, Y); y is (NS, nb)."""
+        y = np.asarray(y, dtype=np.float64)
+        if y.ndim == 1:
+            y = y[:, None]
+        temperature = np.broadcast_to(np.asarray(temperature, dtype=np.float64), y.shape[1:]).copy()
+        w = np.zeros((self.nv, y.shape[1]))
+        w[:self.ns] = y
+        for b in range(y.shape[1]):
+            w[self.ns + 3, b] = self.tb.e_from_T(temperature[b], y[:, b]) + self.tb.rmix(y[:, b]) * temperature[b]
+        return w, temperature
+
+    def integrate(self, w, tg, dttry, time_end, pressure=None):
         """Integrate every reactor over [0, time_end]; w (nv, nb) and tg (nb) updated in place.
+        pressure (nb) given: constant-pressure reactors with w = (Y, 0, 0, 0, h).
         Returns (accepted, rejected) step counts per reactor."""
         t = self.tb
         nb = w.shape[1]
@@ -65,10 +81,12 @@ class ReactorBatch:
         nacc = np.zeros(nb, dtype=np.int32)
         nrej = np.zeros(nb, dtype=np.int32)
         k = self._keep
+        pres = None if pressure is None else np.broadcast_to(np.asarray(pressure, dtype=np.float64), (nb,)).copy()
         rc = self.fn(self.ns, nb, w.ctypes.data, tg.ctypes.data, k[0].ctypes.data, t.nreactions,
                      *(a.ctypes.data for a in k[1:11]), t.num_t_tab, t.t_min_tab, t.dt_tab, R_UNIV, TOL_ITER_NR,
                      k[11].ctypes.data, t.nsetcv, k[12].ctypes.data, t.indx_cp_l, t.indx_cp_r, dttry.ctypes.data,
-                     tend.ctypes.data, self.maxsteps, self.maxtry, self.tol, nacc.ctypes.data, nrej.ctypes.data)
+                     tend.ctypes.data, self.maxsteps, self.maxtry, self.tol, nacc.ctypes.data, nrej.ctypes.data,
+                     None if pres is None else pres.ctypes.data, k[13].ctypes.data, k[14].ctypes.data)
         assert rc == 0
         return nacc, nrej
 
@@ -86,6 +104,13 @@ class ReactorBatch:
             e = w[self.ns + 3, b] / rho
             out[b] = t_from_e(self.tb, e, y)
         return out
+
+
+def t_from_h(tables, h, y, lo=200.0, hi=6000.0):
+    """Root of h(T) = e(T) + R_mix T = h by bracketing, independent of the oracle's Newton."""
+    from scipy.optimize import brentq
+    r = tables.rmix(y)
+    return brentq(lambda t: tables.e_from_T(t, y) + r * t - h, lo, hi, xtol=1e-12, rtol=1e-15, maxiter=200)
 
 
 def t_from_e(tables, e, y, lo=200.0, hi=6000.0):

@@ -571,3 +571,98 @@ def test_weno_shock_tube_against_exact_riemann(pkg, be, flux_splitting):
     xs_num = x[np.nonzero(rho > pre + 0.5 * (post - pre))[0].max()]
     xs_ref = x[np.nonzero(ref > pre + 0.5 * (post - pre))[0].max()]
     assert abs(xs_num - xs_ref) <= 2 * length / n
+
+
+# ---------------------------------------------------------------- constant-pressure reactor
+def _h2_air_stoich(tb):
+    names = [n.upper() for n in tb.names]
+    x0 = np.zeros(tb.n_species)
+    x0[names.index("H2")], x0[names.index("O2")], x0[names.index("N2")] = 2.0, 1.0, 3.76
+    return x0 * tb.mw / (x0 * tb.mw).sum()
+
+
+def test_constant_pressure_reactor_inverts_enthalpy(pkg, be):
+    """Inert reactors (pure N2: every rate of progress is zero) keep their state bit for bit,
+    and the temperature the integrator inverts from h is the bracketing root of h(T)."""
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    k = [n.upper() for n in tb.names].index("N2")
+    # (not exactly at the 1000 K range boundary, where N2's two NASA-7 fits differ in h by
+    # 1.6e-4 K worth and the inversion settles in the other fit)
+    temps = np.array([300.0, 812.5, 999.5, 1000.5, 1733.3, 2500.0, 4100.0])
+    y = np.zeros((ns, temps.size))
+    y[k] = 1.0
+    R = reactor_util.ReactorBatch(be.lib, tb, tol=1e-8)
+    w, _ = R.state_cp(temps, y)
+    w0 = w.copy()
+    tg = np.full(temps.size, 1200.0)
+    R.integrate(w, tg, 1e-9, 1e-6, pressure=np.full(temps.size, 101325.0))
+    np.testing.assert_array_equal(w, w0)
+    np.testing.assert_allclose(tg, temps, rtol=1e-12)
+
+
+def test_constant_pressure_reactor_initial_rates_match_constant_volume(pkg, be):
+    """Over a horizon short enough that T and rho barely move, a constant-pressure reactor's
+    change of rho0*Y equals the constant-volume reactor's change of rho*Y from the same state
+    (both are rho0 * omega W dt / rho0 to first order): pins rho = p / (R_mix T), the 1/rho of
+    dY/dt and the temperature taken from h."""
+    tb = pkg.cases.h2_air_mechanism().tables()
+    ns = tb.n_species
+    rng = np.random.default_rng(4)
+    nb = 12
+    y = np.repeat(_h2_air_stoich(tb)[:, None], nb, 1) + rng.uniform(0.0, 2e-3, (ns, nb))
+    y = y / y.sum(axis=0)
+    temps = rng.uniform(1100.0, 1800.0, nb)
+    pres = rng.uniform(0.5, 10.0, nb) * 101325.0
+    R = reactor_util.ReactorBatch(be.lib, tb, tol=1e-10)
+    wv, tv = R.state(temps, pres, y)
+    wp, tp = R.state_cp(temps, y)
+    rho0 = wv[:ns].sum(axis=0)
+    dt = 2e-10
+    wv1, wp1 = wv.copy(), wp.copy()
+    R.integrate(wv1, tv.copy(), dt, dt)
+    R.integrate(wp1, tp.copy(), dt, dt, pressure=pres)
+    d_cv = wv1[:ns] - wv[:ns]
+    d_cp = (wp1[:ns] - wp[:ns]) * rho0
+    scale = np.abs(d_cv).max(axis=0)
+    assert (scale > 0).all()
+    assert (np.abs(d_cp - d_cv).max(axis=0) <= 2e-3 * scale).all()
+
+
+def test_constant_pressure_reactor_reaches_hp_equilibrium(pkg, be):
+    """A stoichiometric H2-air reactor at constant pressure integrated to rest reaches the
+    equilibrium at its (h, p): the composition minimising the Gibbs energy at the final T and
+    the given p, at the temperature where that composition has the initial enthalpy (the
+    adiabatic flame temperature), both found on the host by element potentials and bracketing."""
+    from scipy.optimize import brentq
+    mech = pkg.cases.h2_air_mechanism()
+    tb = mech.tables()
+    ns = tb.n_species
+    y0 = _h2_air_stoich(tb)
+    p = 3.0 * 101325.0
+    R = reactor_util.ReactorBatch(be.lib, tb, tol=1e-10, maxsteps=200000)
+    w, tg = R.state_cp(1100.0, y0)
+    h0 = w[ns + 3, 0]
+    R.integrate(w, tg, 1e-9, 0.05, pressure=np.array([p]))
+    y = w[:ns, 0] / w[:ns, 0].sum()
+    t_fin = reactor_util.t_from_h(tb, h0, y)
+    assert tg[0] == pytest.approx(t_fin, rel=1e-9)
+    elements = sorted({e for s in mech.species for e in s.composition})
+    a_el = np.array([[s.composition.get(e, 0.0) for s in mech.species] for e in elements])
+    b_el = a_el @ (y0 / tb.mw)
+
+    def y_eq(t):
+        g_rt = np.array([np.subtract(*mech.h_RT_s_R(k, np.array([t])))[0] for k in range(ns)])
+        x = _equilibrium_moles(g_rt, a_el, b_el, p / 101325.0)
+        return x * tb.mw / (x * tb.mw).sum()
+
+    def dh(t):
+        ye = y_eq(t)
+        return tb.e_from_T(t, ye) + tb.rmix(ye) * t - h0
+    t_ad = brentq(dh, 2000.0, 3500.0, xtol=1e-6)
+    assert t_fin == pytest.approx(t_ad, abs=0.05)
+    x = y / tb.mw / (y / tb.mw).sum()
+    ye = y_eq(t_fin)
+    x_eq = ye / tb.mw / (ye / tb.mw).sum()
+    big = x_eq > 1e-6
+    np.testing.assert_allclose(x[big], x_eq[big], rtol=2e-5)
