@@ -81,6 +81,19 @@ int streams_bcextr(int ilat, int nx, int ny, int nz, int ng, int nv, double *w, 
 int streams_bcwall_ad(int n_s, int ilat, int nx, int ny, int nz, int ng, double *w,
                       void *stream);
 
+/* bc_nr_lat_{x,y,z}_kernel, kernels_gpu.F90:4330-4793 (launched by bc_nr,
+ * multideal_gpu.F90:1116-1163) on face (dir = 1..3, start_or_end = 1 min / 2 max):
+ * characteristic boundary treatment of the face cells, nr_type 1 non-reflecting, 2
+ * relaxation to the ghost cell, 3 relaxation to the freestream state winf(1:nv), 6
+ * reflective wall: fl += R (lambda L dw/dn) * metric(face cell).  The face cells must be
+ * excluded from the convective update (eul_imin = 2 etc., bc_preproc, multideal.F90:5671-
+ * 5718).  metric is dcsidx / detady / dzitdz; winf may be NULL unless nr_type = 3. */
+int streams_bc_nr_lat(int dir, int start_or_end, int nr_type, int n_s, int nx, int ny, int nz,
+                      int ng, int nv, const double *w_aux, const double *w, double *fl,
+                      const double *metric, int indx_cp_l, int indx_cp_r, const double *rgas,
+                      const double *cv_coeff, const double *cp_coeff, int nsetcv,
+                      const double *trange, const double *winf, void *stream);
+
 /* ---- convective flux (a4, a5) ------------------------------------------------------ */
 
 /* euler_x (multideal_gpu.F90:948-1002): euler_x_fluxes_hybrid_kernel
@@ -150,6 +163,17 @@ int streams_visflx_reduced(int n_s, int nx, int ny, int nz, int ng, int nv_aux,
                            const double *coeff_deriv1, const double *dcsidx,
                            const double *detady, const double *dzitdz, void *stream);
 
+/* visflx_cuf, kernels_gpu.F90:3357-3537 (conservative_viscous = 0, mode 0): the Laplacian
+ * form of the viscous terms from cell-centred first and second derivatives (coeff_deriv2(0:4,4),
+ * second-derivative metrics dcsidxs/dcsidx2 etc.), with conduction as (grad mu . grad T +
+ * mu lap T) k/mu; momentum and energy only (no species diffusion in this form). */
+int streams_visflx(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
+                   double *w_aux, double *fl, const double *coeff_deriv1,
+                   const double *coeff_deriv2, const double *dcsidx, const double *detady,
+                   const double *dzitdz, const double *dcsidxs, const double *detadys,
+                   const double *dzitdzs, const double *dcsidx2, const double *detady2,
+                   const double *dzitdz2, void *stream);
+
 /* ---- chemistry (a7, a8) ------------------------------------------------------------ */
 
 /* compute_chemistry_cuf, kernels_gpu.F90:9503-9674: explicit finite-rate source,
@@ -214,6 +238,18 @@ int streams_init_flux(int nx, int ny, int nz, int nv, double *fl, double *fln,
 /* update_flux_cuf (:3297-3314): fln -= gamdt*fl. */
 int streams_update_flux(int nx, int ny, int nz, int nv, const double *fl, double *fln,
                         double gamdt, void *stream);
+/* Simpler operator splitting (operator_splitting = 1, rk_sync(simpler_splitting=1),
+ * multideal_gpu.F90:536-621 and advance_solution :1875-1893): the RK stages accumulate
+ * fl - fl_sav, where fl_sav holds the transport right-hand side saved by sav_flx
+ * (:623-692) and fed to rosenbrock_krylov as its forcing.
+ * init_flux_simpler_cuf (kernels_gpu.F90:47-66): fln = -rhodt*(fl - fl_sav); fl = 0. */
+int streams_init_flux_simpler(int nx, int ny, int nz, int nv, double *fl, double *fln,
+                              const double *fl_sav, double rhodt, void *stream);
+/* sav_flx_cuf (:68-86): fl_sav = fl. */
+int streams_sav_flx(int nx, int ny, int nz, int nv, const double *fl, double *fl_sav, void *stream);
+/* update_simpler_flux_cuf (:3316-3334): fln -= gamdt*(fl - fl_sav). */
+int streams_update_simpler_flux(int nx, int ny, int nz, int nv, const double *fl, double *fln,
+                                const double *fl_sav, double gamdt, void *stream);
 /* update_field_cuf (:3336-3355): w += fln where fluid_mask == 0 (fluid_mask may be NULL
  * for an all-fluid domain). */
 int streams_update_field(int nx, int ny, int nz, int ng, int nv, double *w,
@@ -229,6 +265,50 @@ int streams_compute_dt(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_a
                        int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
                        const double *trange, const int *fluid_mask, void *stream);
 
+
+/* ---- host <-> device field copies and checkpoints (base_gpu.F90:1266-1325) ---------- */
+
+/* copy_cpu_gpu (base_gpu.F90:1266-1286): the host field w_host(nv, 1-ng:nx+ng, 1-ng:ny+ng,
+ * 1-ng:nz+ng) (variable fastest, host memory) into the device field w_gpu(1-ng:..,..,..,
+ * 1:nv); the transposition runs on the device.  nv <= 32.  Waits for the stream. */
+int streams_copy_cpu_gpu(int nx, int ny, int nz, int ng, int nv, const double *w_host,
+                         double *w_gpu, void *stream);
+
+/* copy_gpu_cpu (:1303-1325): the reverse, into host memory.  Waits for the stream. */
+int streams_copy_gpu_cpu(int nx, int ny, int nz, int ng, int nv, const double *w_gpu,
+                         double *w_host, void *stream);
+
+/* The interior w_gpu(1:nx,1:ny,1:nz,1:nv) of a padded device field to a dense device array
+ * dense(nx,ny,nz,nv) (to_dense = 1; the layout of rst.bin, field.F90:427-477) or back
+ * (to_dense = 0).  No counterpart as one routine: the reference writes rst.bin from the host
+ * copy through an MPI-IO subarray view. */
+int streams_field_interior(int nx, int ny, int nz, int ng, int nv, double *w_gpu, double *dense,
+                           int to_dense, void *stream);
+
+/* ---- run-loop monitors (the time loop of run, multideal_gpu.F90:1659-1851) ----------- */
+/* Each writes its results to host memory and waits for the stream.  The reductions combine
+ * partial results in a fixed order, so a result does not depend on the launch. */
+
+/* compute_residual_cuf (kernels_gpu.F90:5991-6012): residual_rhou (host) = sum over fluid
+ * points of (fln(i,j,k,I_U)/dt)^2.  fluid_mask may be NULL. */
+int streams_compute_residual(int nx, int ny, int nz, int ng, int nv, const double *fln, double dt,
+                             double *residual_rhou, const int *fluid_mask, void *stream);
+
+/* compute_rho_t_p_minmax_cuf (:6038-6073): extrema of w_aux(J_R), w_aux(J_T), w_aux(J_P)
+ * over fluid points, written to the six host scalars (minima start at huge(), maxima at
+ * -100, as there). */
+int streams_compute_rho_t_p_minmax(int n_s, int nx, int ny, int nz, int ng, int nv_aux,
+                                   const double *w_aux, double *rhomin, double *rhomax,
+                                   double *tmin, double *tmax, double *pmin, double *pmax,
+                                   const int *fluid_mask, void *stream);
+
+/* find_boom_cuf (:10334-10358): the fluid cells where (fln(i,j,k,2)/dt)^2 is NaN.  The
+ * reference prints each one's global indices (i + ncoords1*nx, j, k + ncoords3*nz); here
+ * *nboom (host) receives their number and first_ijk[3] (host) those indices of the first
+ * one in i-fastest order (zeros when there is none). */
+int streams_find_boom(int nx, int ny, int nz, int ng, int nv, const double *fln, double dt,
+                      const int *fluid_mask, int ncoords1, int ncoords3, int *nboom,
+                      int *first_ijk, void *stream);
 
 /* ---- fused substep stages (MI355X-first orchestration of rk_sync) ------------------- */
 /* These have no single counterpart in the reference: each folds consecutive routines of
@@ -290,10 +370,14 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
  * reached (time_end[b] when done) and the step it would take next.  Kinetics come from the device copy of
  * streams_kinetics_pack.  work is one unsigned of device memory for the shared work
  * counter (zeroed by the call on the stream); wg_per_cu workgroups of 256 lanes per CU form
- * the persistent grid (4 when wg_per_cu <= 0; never more than the batch needs).  mixed = 0 is the FP64 integrator; mixed = 1 solves on the species
- * block (the momentum / energy rows of a reactor's RHS are zero), with a one-sided species
- * Jacobian factored in FP32 and every stage solve refined once against the FP64 residual
- * (same stages, error norm and step control).  No
+ * the persistent grid (4 when wg_per_cu <= 0; never more than the batch needs).
+ * mixed = 0 is the plain FP64 integrator in the reference's arithmetic forms (central-
+ * difference Jacobian over all N_S + 4 variables, FP64 LU, rosenbrock_rhs's forms of the
+ * rate expressions and of the temperature inversion): the yardstick of the other forms.
+ * mixed = 1 solves on the species block (the momentum / energy rows of a reactor's RHS
+ * are zero), with a one-sided species Jacobian factored in FP32 and every stage solve
+ * refined once against the FP64 residual (same stages, error norm and step control);
+ * mixed = 2 is that species-block integrator with an FP64 LU.  No
  * counterpart as one routine in the reference: its rosenbrock_kernel takes the cells of a
  * field, this takes a flat batch. */
 int streams_reactor_batch(int n_s, int nbatch, double *w, double *tguess, const double *dttry,

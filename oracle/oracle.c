@@ -15,6 +15,7 @@
  * Pinning (which check fixes each function) is listed in the test files; functions whose
  * only check is agreement with the GPU path are marked "parity unpinned" there.
  */
+#include <float.h>
 #include <math.h>
 #include <stddef.h>
 #include <stdlib.h>
@@ -63,6 +64,10 @@
 #define J_Z (ns + 12)
 #define J_K_COND (ns + 13)
 #define J_D_START (ns + 13)
+#define J_LES1 (14 + 2 * ns)
+#define J_WDOT_START (14 + 2 * ns)
+#define J_HRR (15 + 3 * ns)
+#define J_GAM_PASR (15 + 3 * ns)
 
 /* tables */
 #define NCOL (indx_cp_r + 2 - indx_cp_l + 1)
@@ -1239,6 +1244,101 @@ int orc_visflx_reduced(int ns, int nx, int ny, int nz, int ng, int nv_aux, int v
     return 0;
 }
 
+/* visflx_cuf (kernels_gpu.F90:3357-3537), conservative_viscous = 0: the Laplacian form of
+ * the viscous terms, momentum and energy only */
+#define CD2(l, lmax) coeff_deriv2[(size_t)(l) + 5 * (size_t)((lmax) - 1)]
+int orc_visflx(int ns, int nx, int ny, int nz, int ng, int nv_aux, int visc_order, double *w_aux, double *fl,
+               const double *coeff_deriv1, const double *coeff_deriv2, const double *dcsidx, const double *detady,
+               const double *dzitdz, const double *dcsidxs, const double *detadys, const double *dzitdzs,
+               const double *dcsidx2, const double *detady2, const double *dzitdz2, void *stream)
+{
+    (void)nv_aux; (void)stream;
+    int lmax = visc_order / 2;
+    OMP_FOR
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        double uu = W(w_aux, i, j, k, J_U), vv = W(w_aux, i, j, k, J_V), ww = W(w_aux, i, j, k, J_W);
+        double tt = W(w_aux, i, j, k, J_T), mu = W(w_aux, i, j, k, J_MU);
+        double k_cond = W(w_aux, i, j, k, J_K_COND), div3l = W(w_aux, i, j, k, J_DIV);
+        double ux = 0, vx = 0, wx = 0, tx = 0, mux = 0, divx3l = 0;
+        double uy = 0, vy = 0, wy = 0, ty = 0, muy = 0, divy3l = 0;
+        double uz = 0, vz = 0, wz = 0, tz = 0, muz = 0, divz3l = 0;
+        double ulapx = CD2(0, lmax) * uu, ulapy = ulapx, ulapz = ulapx;
+        double vlapx = CD2(0, lmax) * vv, vlapy = vlapx, vlapz = vlapx;
+        double wlapx = CD2(0, lmax) * ww, wlapy = wlapx, wlapz = wlapx;
+        double tlapx = CD2(0, lmax) * tt, tlapy = tlapx, tlapz = tlapx;
+        for (int l = 1; l <= lmax; ++l) {
+            double clapl = CD2(l, lmax);
+            ulapx = ulapx + clapl * (W(w_aux, i + l, j, k, J_U) + W(w_aux, i - l, j, k, J_U));
+            ulapy = ulapy + clapl * (W(w_aux, i, j + l, k, J_U) + W(w_aux, i, j - l, k, J_U));
+            ulapz = ulapz + clapl * (W(w_aux, i, j, k + l, J_U) + W(w_aux, i, j, k - l, J_U));
+            vlapx = vlapx + clapl * (W(w_aux, i + l, j, k, J_V) + W(w_aux, i - l, j, k, J_V));
+            vlapy = vlapy + clapl * (W(w_aux, i, j + l, k, J_V) + W(w_aux, i, j - l, k, J_V));
+            vlapz = vlapz + clapl * (W(w_aux, i, j, k + l, J_V) + W(w_aux, i, j, k - l, J_V));
+            wlapx = wlapx + clapl * (W(w_aux, i + l, j, k, J_W) + W(w_aux, i - l, j, k, J_W));
+            wlapy = wlapy + clapl * (W(w_aux, i, j + l, k, J_W) + W(w_aux, i, j - l, k, J_W));
+            wlapz = wlapz + clapl * (W(w_aux, i, j, k + l, J_W) + W(w_aux, i, j, k - l, J_W));
+            tlapx = tlapx + clapl * (W(w_aux, i + l, j, k, J_T) + W(w_aux, i - l, j, k, J_T));
+            tlapy = tlapy + clapl * (W(w_aux, i, j + l, k, J_T) + W(w_aux, i, j - l, k, J_T));
+            tlapz = tlapz + clapl * (W(w_aux, i, j, k + l, J_T) + W(w_aux, i, j, k - l, J_T));
+        }
+        for (int l = 1; l <= lmax; ++l) {
+            double ccl = CD1(l, lmax);
+            ux = ux + ccl * (W(w_aux, i + l, j, k, J_U) - W(w_aux, i - l, j, k, J_U));
+            vx = vx + ccl * (W(w_aux, i + l, j, k, J_V) - W(w_aux, i - l, j, k, J_V));
+            wx = wx + ccl * (W(w_aux, i + l, j, k, J_W) - W(w_aux, i - l, j, k, J_W));
+            tx = tx + ccl * (W(w_aux, i + l, j, k, J_T) - W(w_aux, i - l, j, k, J_T));
+            mux = mux + ccl * (W(w_aux, i + l, j, k, J_MU) - W(w_aux, i - l, j, k, J_MU));
+            divx3l = divx3l + ccl * (W(w_aux, i + l, j, k, J_DIV) - W(w_aux, i - l, j, k, J_DIV));
+            uy = uy + ccl * (W(w_aux, i, j + l, k, J_U) - W(w_aux, i, j - l, k, J_U));
+            vy = vy + ccl * (W(w_aux, i, j + l, k, J_V) - W(w_aux, i, j - l, k, J_V));
+            wy = wy + ccl * (W(w_aux, i, j + l, k, J_W) - W(w_aux, i, j - l, k, J_W));
+            ty = ty + ccl * (W(w_aux, i, j + l, k, J_T) - W(w_aux, i, j - l, k, J_T));
+            muy = muy + ccl * (W(w_aux, i, j + l, k, J_MU) - W(w_aux, i, j - l, k, J_MU));
+            divy3l = divy3l + ccl * (W(w_aux, i, j + l, k, J_DIV) - W(w_aux, i, j - l, k, J_DIV));
+            uz = uz + ccl * (W(w_aux, i, j, k + l, J_U) - W(w_aux, i, j, k - l, J_U));
+            vz = vz + ccl * (W(w_aux, i, j, k + l, J_V) - W(w_aux, i, j, k - l, J_V));
+            wz = wz + ccl * (W(w_aux, i, j, k + l, J_W) - W(w_aux, i, j, k - l, J_W));
+            tz = tz + ccl * (W(w_aux, i, j, k + l, J_T) - W(w_aux, i, j, k - l, J_T));
+            muz = muz + ccl * (W(w_aux, i, j, k + l, J_MU) - W(w_aux, i, j, k - l, J_MU));
+            divz3l = divz3l + ccl * (W(w_aux, i, j, k + l, J_DIV) - W(w_aux, i, j, k - l, J_DIV));
+        }
+        double dx = dcsidx[i - 1], dy = detady[j - 1], dz = dzitdz[k - 1];
+        ux *= dx; vx *= dx; wx *= dx; tx *= dx; mux *= dx; divx3l *= dx;
+        uy *= dy; vy *= dy; wy *= dy; ty *= dy; muy *= dy; divy3l *= dy;
+        uz *= dz; vz *= dz; wz *= dz; tz *= dz; muz *= dz; divz3l *= dz;
+        ulapx = ulapx * dcsidxs[i - 1] + ux * dcsidx2[i - 1];
+        vlapx = vlapx * dcsidxs[i - 1] + vx * dcsidx2[i - 1];
+        wlapx = wlapx * dcsidxs[i - 1] + wx * dcsidx2[i - 1];
+        tlapx = tlapx * dcsidxs[i - 1] + tx * dcsidx2[i - 1];
+        ulapy = ulapy * detadys[j - 1] + uy * detady2[j - 1];
+        vlapy = vlapy * detadys[j - 1] + vy * detady2[j - 1];
+        wlapy = wlapy * detadys[j - 1] + wy * detady2[j - 1];
+        tlapy = tlapy * detadys[j - 1] + ty * detady2[j - 1];
+        ulapz = ulapz * dzitdzs[k - 1] + uz * dzitdz2[k - 1];
+        vlapz = vlapz * dzitdzs[k - 1] + vz * dzitdz2[k - 1];
+        wlapz = wlapz * dzitdzs[k - 1] + wz * dzitdz2[k - 1];
+        tlapz = tlapz * dzitdzs[k - 1] + tz * dzitdz2[k - 1];
+        double ulap = ulapx + ulapy + ulapz, vlap = vlapx + vlapy + vlapz;
+        double wlap = wlapx + wlapy + wlapz, tlap = tlapx + tlapy + tlapz;
+        double sig11 = 2.0 * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
+        double sig22 = 2.0 * (vy - div3l), sig23 = vz + wy, sig33 = 2.0 * (wz - div3l);
+        double sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * (ulap + divx3l);
+        double sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * (vlap + divy3l);
+        double sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * (wlap + divz3l);
+        double sigqt = (mux * tx + muy * ty + muz * tz + mu * tlap) * k_cond / mu;
+        double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz + sig13 * wx +
+                        sig23 * wy + sig33 * wz) * mu;
+        double sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
+        F(fl, i, j, k, I_U) = F(fl, i, j, k, I_U) - sigx;
+        F(fl, i, j, k, I_V) = F(fl, i, j, k, I_V) - sigy;
+        F(fl, i, j, k, I_W) = F(fl, i, j, k, I_W) - sigz;
+        F(fl, i, j, k, I_E) = F(fl, i, j, k, I_E) - sigq;
+    }
+    return 0;
+}
+
 /* ===================================================================================== */
 /* a7: explicit chemistry                                                                 */
 /* ===================================================================================== */
@@ -1402,7 +1502,47 @@ typedef struct {
     /* constant-pressure reactor (pres > 0): state (Y_1..Y_NS, 0, 0, 0, h) */
     const double *cp_coeff, *rgas;
     double pres;
+    /* PaSR closure of the LES path (pasr != 0): the cell's mixing time tau_k */
+    int pasr;
+    double tau_k;
 } chem_t;
+
+/* The PaSR factor of rosenbrock_rhs (kernels_gpu.F90:9359-9379) and of
+ * rosenbrock_krylov_rhs (:10954-10974): the chemical time scale tau_c is the geometric mean
+ * of the slowest and the fastest species time Y_s / |wdot_s / rho| over the species with
+ * |wdot_s / rho| > 1e-8, and the species sources are scaled by tau_c / (tau_c + tau_k)
+ * (by 1 when tau_k is not above the smallest normal double). */
+static void pasr_scale(int ns, double tau_k, double ri, const double *yy, double *rhs)
+{
+    double tau_cf = 0.0, tau_cs = DBL_MAX;
+    for (int lsp = 0; lsp < ns; ++lsp) {
+        if (fabs(rhs[lsp] * ri) > 1.0e-8) {
+            double tau_lsp = yy[lsp] / fabs(rhs[lsp] * ri);
+            if (tau_lsp > tau_cf) tau_cf = tau_lsp;
+            if (tau_lsp > 0.0 && tau_lsp < tau_cs) tau_cs = tau_lsp;
+        }
+    }
+    double tau_c = sqrt(tau_cs * tau_cf);
+    double gam_pasr = tau_k > DBL_MIN ? tau_c / (tau_c + tau_k) : 1.0;
+    for (int lsp = 0; lsp < ns; ++lsp) rhs[lsp] = gam_pasr * rhs[lsp];
+}
+
+/* The cell's mixing time of the PaSR closure (kernels_gpu.F90:8970-8980, 9720-9728):
+ * Delta = (dxi/dx deta/dy dzeta/dz)^(-1/3), nu_sgs = mu_sgs / rho,
+ * eps_sgs = c_eps c_yoshi^(3/2) nu_sgs^3 / Delta^4, tau_k = c_mix sqrt(nu_eff / eps_sgs)
+ * with nu_eff = mu / rho (mu already holds mu_sgs). */
+static double pasr_tau_k(double rho, double mu_sgs, double mu, double dx, double dy, double dz,
+                         double les_c_yoshi, double les_c_mix, double les_c_eps)
+{
+    double ri = 1.0 / rho;
+    double delta = pow(dx * dy * dz, -1.0 / 3.0);
+    double nu_sgs = mu_sgs * ri;
+    double d2 = delta * delta;
+    double eps_sgs = les_c_eps * pow(les_c_yoshi, 1.5) * (nu_sgs * nu_sgs * nu_sgs) / (d2 * d2);
+    double nu_eff = mu * ri;
+    double r = nu_eff / (eps_sgs + DBL_MIN);
+    return les_c_mix * sqrt(r > 0.0 ? r : 0.0);
+}
 
 static void ros_rhs_cv(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs);
 static void ros_rhs_cp(const chem_t *C, double *wv, double *yy, double *tguess, double *rhs);
@@ -1505,14 +1645,18 @@ static void ros_rhs_cv(const chem_t *C, double *wv, double *yy, double *tguess, 
             }
         }
     }
+    if (C->pasr) pasr_scale(ns, C->tau_k, ri, yy, rhs);
 }
 
-/* ludcmp (kernels_gpu.F90:9395-9462), Crout with implicit partial pivoting */
+/* ludcmp (kernels_gpu.F90:9395-9462), Crout with implicit partial pivoting.  A row whose
+ * largest magnitude is below tiny (or NaN) ends it early, as in the reference, which leaves
+ * INDX unset there; indx starts as the identity so that lubksb stays inside its vector. */
 static void ludcmp(int n, double *a, int *indx)
 {
     const double tiny = 1.5e-16;
     double vv[NVMAX];
 #define A(r, c) a[((r) - 1) + (size_t)n * ((c) - 1)]
+    for (int i = 1; i <= n; ++i) indx[i - 1] = i;
     for (int i = 1; i <= n; ++i) {
         double amax = 0.0;
         for (int j = 1; j <= n; ++j) if (fabs(A(i, j)) > amax) amax = fabs(A(i, j));
@@ -1684,9 +1828,8 @@ int orc_rosenbrock(int ns, int nv, int nv_aux, int nx, int ny, int nz, int ng,
                    double les_c_mix, double les_c_eps, const double *dcsidx,
                    const double *detady, const double *dzitdz, void *stream)
 {
-    (void)nv_aux; (void)fl; (void)simpler_splitting; (void)les_c_yoshi; (void)les_c_mix; (void)les_c_eps;
-    (void)dcsidx; (void)detady; (void)dzitdz; (void)stream;
-    if (enable_pasr != 0) return -3;
+    (void)fl; (void)simpler_splitting; (void)stream;
+    if (enable_pasr != 0 && nv_aux < J_LES1) return -2;
     chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
                 falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
                 tol_iter_nr, cv_coeff, trange};
@@ -1698,7 +1841,13 @@ int orc_rosenbrock(int ns, int nv, int nv_aux, int nx, int ny, int nz, int ng,
         for (int m = 1; m <= nv; ++m) wv[m - 1] = W(w, i, j, k, m);
         double tg = W(w_aux, i, j, k, J_T);
         int na, nr;
-        orc_rosenbrock_cell(&C, wv, &tg, dttry, time_start, time_end, maxsteps, maxtry, tol, &na, &nr);
+        chem_t Cc = C;
+        if (enable_pasr > 0) {
+            Cc.pasr = 1;
+            Cc.tau_k = pasr_tau_k(W(w_aux, i, j, k, J_R), W(w_aux, i, j, k, J_LES1), W(w_aux, i, j, k, J_MU),
+                                  dcsidx[i - 1], detady[j - 1], dzitdz[k - 1], les_c_yoshi, les_c_mix, les_c_eps);
+        }
+        orc_rosenbrock_cell(&Cc, wv, &tg, dttry, time_start, time_end, maxsteps, maxtry, tol, &na, &nr);
         for (int m = 1; m <= nv; ++m) W(w, i, j, k, m) = wv[m - 1];
         /* the kernel leaves w_aux T at the last right-hand-side evaluation (w_aux Y,
          * also scratch there, is rebuilt by the next eval_aux and not written here) */
@@ -1782,9 +1931,9 @@ static double kry_norm2(int n, const double *v)
     return sqrt(s);
 }
 
-/* rosenbrock_krylov_rhs (kernels_gpu.F90:10794-10976) with the PaSR factor off: the
- * arithmetic of rosenbrock_rhs on a copy of the state (the species are clipped in the copy
- * only) with the temperature guess passed by value (intent(in) there) */
+/* rosenbrock_krylov_rhs (kernels_gpu.F90:10794-10976): the arithmetic of rosenbrock_rhs
+ * (PaSR factor included) on a copy of the state (the species are clipped in the copy only)
+ * with the temperature guess passed by value (intent(in) there) */
 static void kry_rhs(const chem_t *C, const double *w, double t_iter, double *rhs)
 {
     double tmp[NVMAX], yy[NSMAX];
@@ -1997,9 +2146,8 @@ int orc_rosenbrock_krylov(int ns, int nv, int nv_aux, int nx, int ny, int nz, in
                           int enable_pasr, double les_c_yoshi, double les_c_mix, double les_c_eps,
                           const double *dcsidx, const double *detady, const double *dzitdz, void *stream)
 {
-    (void)nv_aux; (void)fl; (void)les_c_yoshi; (void)les_c_mix; (void)les_c_eps;
-    (void)dcsidx; (void)detady; (void)dzitdz; (void)stream;
-    if (enable_pasr != 0) return -3;
+    (void)fl; (void)stream;
+    if (enable_pasr != 0 && nv_aux < J_LES1) return -2;
     if (simpler_splitting == 1 && fl_sav == NULL) return -1;
     chem_t C = {ns, nv, nreactions, num_t_tab, nsetcv, indx_cp_l, indx_cp_r, mw, arr_a, arr_b, arr_ea, tb_eff,
                 falloff_coeffs, r_coeffs, p_coeffs, kc_tab, reac_ty, isrev, t_min_tab, dt_tab, r_univ,
@@ -2016,7 +2164,13 @@ int orc_rosenbrock_krylov(int ns, int nv, int nv_aux, int nx, int ny, int nz, in
         if (simpler_splitting == 1)
             for (int m = 1; m <= nv; ++m)
                 fls[m - 1] = fl_sav[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)) + fs * (m - 1)];
-        orc_rosenbrock_krylov_cell(&C, vec, W(w_aux, i, j, k, J_T), dttry, time_start, time_end, maxsteps,
+        chem_t Cc = C;
+        if (enable_pasr > 0) {
+            Cc.pasr = 1;
+            Cc.tau_k = pasr_tau_k(W(w_aux, i, j, k, J_R), W(w_aux, i, j, k, J_LES1), W(w_aux, i, j, k, J_MU),
+                                  dcsidx[i - 1], detady[j - 1], dzitdz[k - 1], les_c_yoshi, les_c_mix, les_c_eps);
+        }
+        orc_rosenbrock_krylov_cell(&Cc, vec, W(w_aux, i, j, k, J_T), dttry, time_start, time_end, maxsteps,
                                    simpler_splitting == 1 ? fls : NULL, NULL);
         for (int m = 1; m <= nv; ++m) W(w, i, j, k, m) = vec[m - 1];
     }
@@ -2071,6 +2225,38 @@ int orc_update_flux(int nx, int ny, int nz, int nv, const double *fl, double *fl
     return 0;
 }
 
+/* init_flux_simpler_cuf, kernels_gpu.F90:47-66: fln = -rhodt*(fl - fl_sav); fl = 0 */
+int orc_init_flux_simpler(int nx, int ny, int nz, int nv, double *fl, double *fln, const double *fl_sav,
+                          double rhodt, void *stream)
+{
+    (void)stream;
+    long n = (long)nx * ny * nz * nv;
+    OMP_FOR
+    for (long q = 0; q < n; ++q) { fln[q] = -rhodt * (fl[q] - fl_sav[q]); fl[q] = 0.0; }
+    return 0;
+}
+
+/* sav_flx_cuf, kernels_gpu.F90:68-86: fl_sav = fl */
+int orc_sav_flx(int nx, int ny, int nz, int nv, const double *fl, double *fl_sav, void *stream)
+{
+    (void)stream;
+    long n = (long)nx * ny * nz * nv;
+    OMP_FOR
+    for (long q = 0; q < n; ++q) fl_sav[q] = fl[q];
+    return 0;
+}
+
+/* update_simpler_flux_cuf, kernels_gpu.F90:3316-3334: fln -= gamdt*(fl - fl_sav) */
+int orc_update_simpler_flux(int nx, int ny, int nz, int nv, const double *fl, double *fln, const double *fl_sav,
+                            double gamdt, void *stream)
+{
+    (void)stream;
+    long n = (long)nx * ny * nz * nv;
+    OMP_FOR
+    for (long q = 0; q < n; ++q) fln[q] = fln[q] - gamdt * (fl[q] - fl_sav[q]);
+    return 0;
+}
+
 int orc_update_field(int nx, int ny, int nz, int ng, int nv, double *w, const double *fln,
                      const int *fluid_mask, void *stream)
 {
@@ -2111,5 +2297,456 @@ int orc_compute_dt(int ns, int nx, int ny, int nz, int ng, int nv, int nv_aux,
         for (int q = 0; q < 9; ++q) if (v[q] > mx[q]) mx[q] = v[q];
     }
     for (int q = 0; q < 9; ++q) dtmax9[q] = mx[q];
+    return 0;
+}
+
+/* ===================================================================================== */
+/* run-loop monitors                                                                       */
+/* ===================================================================================== */
+/* compute_residual_cuf (kernels_gpu.F90:5991-6012): sum over fluid points of
+ * (fln(i,j,k,I_U)/dt)^2 */
+int orc_compute_residual(int nx, int ny, int nz, int ng, int nv, const double *fln, double dt,
+                         double *residual_rhou, const int *fluid_mask, void *stream)
+{
+    (void)stream;
+    int ns = nv - 4;
+    double res = 0.0;
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        if (fluid_mask && fluid_mask[PIDX(i, j, k)] != 0) continue;
+        double r = F(fln, i, j, k, I_U) / dt;
+        res = res + r * r;
+    }
+    *residual_rhou = res;
+    return 0;
+}
+
+/* compute_rho_t_p_minmax_cuf (:6038-6073) */
+int orc_compute_rho_t_p_minmax(int ns, int nx, int ny, int nz, int ng, int nv_aux, const double *w_aux,
+                               double *rhomin, double *rhomax, double *tmin, double *tmax, double *pmin,
+                               double *pmax, const int *fluid_mask, void *stream)
+{
+    (void)nv_aux; (void)stream;
+    double v[6] = {1.7976931348623157e308, -100.0, 1.7976931348623157e308, -100.0, 1.7976931348623157e308, -100.0};
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        if (fluid_mask && fluid_mask[PIDX(i, j, k)] != 0) continue;
+        double rho = W(w_aux, i, j, k, J_R), tt = W(w_aux, i, j, k, J_T), pp = W(w_aux, i, j, k, J_P);
+        v[0] = fmin(v[0], rho); v[1] = fmax(v[1], rho);
+        v[2] = fmin(v[2], tt);  v[3] = fmax(v[3], tt);
+        v[4] = fmin(v[4], pp);  v[5] = fmax(v[5], pp);
+    }
+    *rhomin = v[0]; *rhomax = v[1]; *tmin = v[2]; *tmax = v[3]; *pmin = v[4]; *pmax = v[5];
+    return 0;
+}
+
+/* find_boom_cuf (:10334-10358): cells whose (fln(i,j,k,2)/dt)^2 is NaN; the reference prints
+ * each one's global indices, this returns their count and the first in i-fastest order */
+int orc_find_boom(int nx, int ny, int nz, int ng, int nv, const double *fln, double dt, const int *fluid_mask,
+                  int ncoords1, int ncoords3, int *nboom, int *first_ijk, void *stream)
+{
+    (void)nv; (void)stream;
+    int cnt = 0;
+    first_ijk[0] = first_ijk[1] = first_ijk[2] = 0;
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        if (fluid_mask && fluid_mask[PIDX(i, j, k)] != 0) continue;
+        double r = F(fln, i, j, k, 2) / dt;
+        double res = r * r;
+        if (res != res) {
+            if (cnt == 0) { first_ijk[0] = i + ncoords1 * nx; first_ijk[1] = j; first_ijk[2] = k + ncoords3 * nz; }
+            ++cnt;
+        }
+    }
+    *nboom = cnt;
+    return 0;
+}
+
+/* ===================================================================================== */
+/* host <-> device field copies (base_gpu.F90:1266-1325): on the CPU both sides are host   */
+/* arrays, so these are the reference's transposition loops                               */
+/* ===================================================================================== */
+int orc_copy_cpu_gpu(int nx, int ny, int nz, int ng, int nv, const double *w_host, double *w_gpu, void *stream)
+{
+    (void)stream;
+    for (int k = 1 - ng; k <= nz + ng; ++k)
+    for (int j = 1 - ng; j <= ny + ng; ++j)
+    for (int i = 1 - ng; i <= nx + ng; ++i)
+    for (int iv = 1; iv <= nv; ++iv)
+        W(w_gpu, i, j, k, iv) = w_host[(size_t)(iv - 1) + (size_t)nv * PIDX(i, j, k)];
+    return 0;
+}
+
+int orc_copy_gpu_cpu(int nx, int ny, int nz, int ng, int nv, const double *w_gpu, double *w_host, void *stream)
+{
+    (void)stream;
+    for (int k = 1 - ng; k <= nz + ng; ++k)
+    for (int j = 1 - ng; j <= ny + ng; ++j)
+    for (int i = 1 - ng; i <= nx + ng; ++i)
+    for (int iv = 1; iv <= nv; ++iv)
+        w_host[(size_t)(iv - 1) + (size_t)nv * PIDX(i, j, k)] = W(w_gpu, i, j, k, iv);
+    return 0;
+}
+
+int orc_field_interior(int nx, int ny, int nz, int ng, int nv, double *w_gpu, double *dense, int to_dense,
+                       void *stream)
+{
+    (void)stream;
+    for (int m = 1; m <= nv; ++m)
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        if (to_dense) F(dense, i, j, k, m) = W(w_gpu, i, j, k, m);
+        else W(w_gpu, i, j, k, m) = F(dense, i, j, k, m);
+    }
+    return 0;
+}
+
+/* ===================================================================================== */
+/* non-reflecting / relaxation / reflective-wall boundary treatment                      */
+/* ===================================================================================== */
+/* bc_nr_lat_{x,y,z}_kernel (kernels_gpu.F90:4330-4793) for face (dir, start_or_end):
+ * one-sided second-order derivative of the conserved variables normal to the face,
+ * projected on the characteristic fields of the face cell (eigenvectors_{x,y,z}), the
+ * incoming waves removed (nr_type 1), taken from the outer difference to the ghost cell
+ * (2) or to the freestream winf (3), or reflected (6), and the result added to fl of the
+ * face cell: fl += R (lambda L dw/dn) * metric.  The face cells are excluded from the
+ * convective update (bc_preproc, multideal.F90:5671-5718). */
+int orc_bc_nr_lat(int dir, int start_or_end, int nr_type, int ns, int nx, int ny, int nz, int ng, int nv,
+                  const double *w_aux, const double *w, double *fl, const double *metric, int indx_cp_l,
+                  int indx_cp_r, const double *rgas, const double *cv_coeff, const double *cp_coeff, int nsetcv,
+                  const double *trange, const double *winf, void *stream)
+{
+    (void)cp_coeff; (void)stream;
+    if (nv != ns + 4 || dir < 1 || dir > 3 || (start_or_end != 1 && start_or_end != 2)) return -2;
+    if (nr_type != 1 && nr_type != 2 && nr_type != 3 && nr_type != 6) return -3;
+    if ((dir == 1 ? nx : dir == 2 ? ny : nz) < 3 || (nr_type == 3 && !winf)) return -2;
+    const double c_one[3] = {-1.5, 2.0, -0.5};
+    const int sgn = start_or_end == 1 ? 1 : -1;
+    int n1 = dir == 1 ? ny : nx, n2 = dir == 3 ? ny : nz;
+    for (int b2 = 1; b2 <= n2; ++b2)
+    for (int b1 = 1; b1 <= n1; ++b1) {
+        int i, j, k;
+        int nb = dir == 1 ? nx : dir == 2 ? ny : nz;
+        int bnd = start_or_end == 1 ? 1 : nb;
+        if (dir == 1) { i = bnd; j = b1; k = b2; }
+        else if (dir == 2) { i = b1; j = bnd; k = b2; }
+        else { i = b1; j = b2; k = bnd; }
+        int di = dir == 1, dj = dir == 2, dk = dir == 3;
+        double dw_dn[NVMAX], dw_dn_outer[NVMAX], dwc_dn[NVMAX], dwc_dn_outer[NVMAX], ev[NVMAX];
+        double yyvec[NSMAX], prhoi[NSMAX], el[NVMAX * NVMAX], er[NVMAX * NVMAX];
+        for (int m = 1; m <= nv; ++m) {
+            dw_dn[m - 1] = 0.0;
+            for (int l = 1; l <= 3; ++l)
+                dw_dn[m - 1] = dw_dn[m - 1] + sgn * c_one[l - 1] *
+                               W(w, i + di * sgn * (l - 1), j + dj * sgn * (l - 1), k + dk * sgn * (l - 1), m);
+            double w_target = 0.0;
+            if (nr_type == 2) w_target = W(w, i - di * sgn, j - dj * sgn, k - dk * sgn, m);
+            else if (nr_type == 3) w_target = winf[m - 1];
+            dw_dn_outer[m - 1] = sgn * (W(w, i, j, k, m) - w_target);
+        }
+        for (int lsp = 1; lsp <= ns; ++lsp) yyvec[lsp - 1] = W(w_aux, i, j, k, lsp);
+        double rho = W(w_aux, i, j, k, J_R), uu = W(w_aux, i, j, k, J_U), vv = W(w_aux, i, j, k, J_V);
+        double ww = W(w_aux, i, j, k, J_W), h = W(w_aux, i, j, k, J_H), tt = W(w_aux, i, j, k, J_T);
+        double pp = W(w_aux, i, j, k, J_P), c = W(w_aux, i, j, k, J_C);
+        double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        double cc = c * c, rmix_tt = pp / rho, gamloc = cc / rmix_tt, gm1loc = gamloc - 1.0, ci = 1.0 / c;
+        double p_e = rho * gm1loc, etot = h - rmix_tt;
+        double b3 = etot, b2 = p_e / (rho * cc), b1v = -b2 * (etot - 2.0 * qq);
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            double eei = species_h(ns, tt, indx_cp_l, indx_cp_r, cv_coeff, nsetcv, trange, lsp);
+            prhoi[lsp - 1] = rgas[lsp - 1] * tt + gm1loc * (etot - qq - eei);
+        }
+        eigenvectors(dir, ns, b1v, b2, b3, uu, vv, ww, c, ci, h, yyvec, prhoi, el, er);
+        for (int m = 1; m <= nv; ++m) {
+            dwc_dn[m - 1] = 0.0;
+            dwc_dn_outer[m - 1] = 0.0;
+            for (int mm = 1; mm <= nv; ++mm) {
+                dwc_dn[m - 1] = dwc_dn[m - 1] + el[(mm - 1) + (size_t)nv * (m - 1)] * dw_dn[mm - 1];
+                dwc_dn_outer[m - 1] = dwc_dn_outer[m - 1] + el[(mm - 1) + (size_t)nv * (m - 1)] * dw_dn_outer[mm - 1];
+            }
+        }
+        double un = dir == 1 ? uu : dir == 2 ? vv : ww;
+        ev[0] = un - c;
+        for (int lsp = 2; lsp <= ns + 3; ++lsp) ev[lsp - 1] = un;
+        ev[nv - 1] = un + c;
+        if (nr_type == 1)
+            for (int l = 1; l <= nv; ++l) ev[l - 1] = sgn * fmin(sgn * ev[l - 1], 0.0);
+        if (nr_type == 2 || nr_type == 3)
+            for (int m = 1; m <= nv; ++m)
+                if (sgn * ev[m - 1] > 0.0) dwc_dn[m - 1] = dwc_dn_outer[m - 1];
+        for (int m = 1; m <= nv; ++m) dwc_dn[m - 1] = ev[m - 1] * dwc_dn[m - 1];
+        if (nr_type == 6) {
+            for (int lsp = 2; lsp <= ns + 3; ++lsp) dwc_dn[lsp - 1] = 0.0;
+            if (start_or_end == 1) dwc_dn[nv - 1] = dwc_dn[0];
+            else dwc_dn[0] = dwc_dn[nv - 1];
+        }
+        double met = metric[(dir == 1 ? i : dir == 2 ? j : k) - 1];
+        for (int m = 1; m <= nv; ++m) {
+            double df = 0.0;
+            for (int mm = 1; mm <= nv; ++mm) df = df + er[(mm - 1) + (size_t)nv * (m - 1)] * dwc_dn[mm - 1];
+            F(fl, i, j, k, m) = F(fl, i, j, k, m) + df * met;
+        }
+    }
+    return 0;
+}
+
+/* ===================================================================================== */
+/* f4: LES auxiliaries (compute_aux_les, multideal_gpu.F90:445-498)                        */
+/* ===================================================================================== */
+/* eval_velaux_cuf (kernels_gpu.F90:9879-9917): over the padded box, species clipped at 0
+ * in w, Y, rho, u, v, w into w_aux and the SGS viscosity slot cleared. */
+int orc_eval_velaux(int ns, int nx, int ny, int nz, int ng, int nv, int nv_aux, double *w, double *w_aux,
+                    void *stream)
+{
+    (void)stream;
+    if (nv != ns + 4 || nv_aux < J_LES1) return -2;
+    OMP_FOR
+    for (int k = 1 - ng; k <= nz + ng; ++k)
+    for (int j = 1 - ng; j <= ny + ng; ++j)
+    for (int i = 1 - ng; i <= nx + ng; ++i) {
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            double v = W(w, i, j, k, lsp);
+            W(w, i, j, k, lsp) = v > 0.0 ? v : 0.0;
+        }
+        double rho = W(w, i, j, k, 1);
+        for (int lsp = 2; lsp <= ns; ++lsp) rho = rho + W(w, i, j, k, lsp);
+        double ri = 1.0 / rho;
+        for (int lsp = 1; lsp <= ns; ++lsp) W(w_aux, i, j, k, lsp) = W(w, i, j, k, lsp) * ri;
+        W(w_aux, i, j, k, J_R) = rho;
+        W(w_aux, i, j, k, J_U) = W(w, i, j, k, I_U) * ri;
+        W(w_aux, i, j, k, J_V) = W(w, i, j, k, I_V) * ri;
+        W(w_aux, i, j, k, J_W) = W(w, i, j, k, I_W) * ri;
+        W(w_aux, i, j, k, J_LES1) = 0.0;
+    }
+    return 0;
+}
+
+/* les_wale_mut_cuf (kernels_gpu.F90:9921-10047): on the interior, the velocity gradient by
+ * central differences of order ep_order, the Ducros sensor and div/3 (as eval_aux2 does),
+ * and the WALE eddy viscosity
+ *   mu_sgs = rho C_w^2 Delta^2 |S^d|^3 / (|S|^5 + |S^d|^(5/2) + tiny),  Delta^2 = (dxi/dx deta/dy dzeta/dz)^(-2/3),
+ * S the strain rate, S^d the traceless symmetric part of the squared velocity gradient. */
+int orc_les_wale_mut(int ns, int nx, int ny, int nz, int ng, int nv_aux, int ep_order, double *w_aux,
+                     const double *coeff_deriv1, const double *dcsidx, const double *detady, const double *dzitdz,
+                     double les_c_wale, double eps_sensor, int sensor_type, void *stream)
+{
+    (void)stream;
+    int lmax = ep_order / 2;
+    if (nv_aux < J_LES1) return -2;
+    if (lmax < 1 || lmax > 4 || lmax > ng) return -3;
+    OMP_FOR
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        double delta2 = dcsidx[i - 1] * detady[j - 1] * dzitdz[k - 1];
+        delta2 = pow(delta2, -2.0 / 3.0);
+        double dudx = 0, dudy = 0, dudz = 0, dvdx = 0, dvdy = 0, dvdz = 0, dwdx = 0, dwdy = 0, dwdz = 0;
+        for (int l = 1; l <= lmax; ++l) {
+            double ccl = CD1(l, lmax);
+            dudx = dudx + ccl * (W(w_aux, i + l, j, k, J_U) - W(w_aux, i - l, j, k, J_U));
+            dudy = dudy + ccl * (W(w_aux, i, j + l, k, J_U) - W(w_aux, i, j - l, k, J_U));
+            dudz = dudz + ccl * (W(w_aux, i, j, k + l, J_U) - W(w_aux, i, j, k - l, J_U));
+            dvdx = dvdx + ccl * (W(w_aux, i + l, j, k, J_V) - W(w_aux, i - l, j, k, J_V));
+            dvdy = dvdy + ccl * (W(w_aux, i, j + l, k, J_V) - W(w_aux, i, j - l, k, J_V));
+            dvdz = dvdz + ccl * (W(w_aux, i, j, k + l, J_V) - W(w_aux, i, j, k - l, J_V));
+            dwdx = dwdx + ccl * (W(w_aux, i + l, j, k, J_W) - W(w_aux, i - l, j, k, J_W));
+            dwdy = dwdy + ccl * (W(w_aux, i, j + l, k, J_W) - W(w_aux, i, j - l, k, J_W));
+            dwdz = dwdz + ccl * (W(w_aux, i, j, k + l, J_W) - W(w_aux, i, j, k - l, J_W));
+        }
+        dudx *= dcsidx[i - 1]; dudy *= detady[j - 1]; dudz *= dzitdz[k - 1];
+        dvdx *= dcsidx[i - 1]; dvdy *= detady[j - 1]; dvdz *= dzitdz[k - 1];
+        dwdx *= dcsidx[i - 1]; dwdy *= detady[j - 1]; dwdz *= dzitdz[k - 1];
+        double div = dudx + dvdy + dwdz;
+        double div3l = div / 3.0;
+        double omegax = dwdy - dvdz, omegay = dudz - dwdx, omegaz = dvdx - dudy;
+        double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+        double s = -div / sqrt(omod2 + div * div + eps_sensor);
+        if (sensor_type != 0 && s < 0.0) s = 0.0;
+        W(w_aux, i, j, k, J_DUC) = s * s;
+        double S_xx = dudx, S_xy = 0.5 * (dudy + dvdx), S_xz = 0.5 * (dudz + dwdx);
+        double S_yy = dvdy, S_yz = 0.5 * (dvdz + dwdy), S_zz = dwdz;
+        double g_xx = dudx * dudx + dudy * dvdx + dudz * dwdx;
+        double g_xy = dudx * dudy + dudy * dvdy + dudz * dwdy;
+        double g_xz = dudx * dudz + dudy * dvdz + dudz * dwdz;
+        double g_yx = dvdx * dudx + dvdy * dvdx + dvdz * dwdx;
+        double g_yy = dvdx * dudy + dvdy * dvdy + dvdz * dwdy;
+        double g_yz = dvdx * dudz + dvdy * dvdz + dvdz * dwdz;
+        double g_zx = dwdx * dudx + dwdy * dvdx + dwdz * dwdx;
+        double g_zy = dwdx * dudy + dwdy * dvdy + dwdz * dwdy;
+        double g_zz = dwdx * dudz + dwdy * dvdz + dwdz * dwdz;
+        double tce = (g_xx + g_yy + g_zz) / 3.0;
+        double Sd_xx = g_xx - tce, Sd_xy = 0.5 * (g_xy + g_yx), Sd_xz = 0.5 * (g_xz + g_zx);
+        double Sd_yy = g_yy - tce, Sd_yz = 0.5 * (g_yz + g_zy), Sd_zz = g_zz - tce;
+        double normS = S_xx * S_xx + S_yy * S_yy + S_zz * S_zz + 2.0 * (S_xy * S_xy + S_xz * S_xz + S_yz * S_yz);
+        normS = sqrt(normS);
+        double normSd = Sd_xx * Sd_xx + Sd_yy * Sd_yy + Sd_zz * Sd_zz +
+                        2.0 * (Sd_xy * Sd_xy + Sd_xz * Sd_xz + Sd_yz * Sd_yz);
+        normSd = sqrt(normSd);
+        double rho = W(w_aux, i, j, k, J_R);
+        double nS2 = normS * normS, nSd2 = normSd * normSd;
+        double mu_sgs = rho * les_c_wale * les_c_wale * delta2 * (nSd2 * normSd) /
+                        (nS2 * nS2 * normS + sqrt(nSd2 * nSd2 * normSd) + DBL_MIN);
+        W(w_aux, i, j, k, J_LES1) = mu_sgs;
+        W(w_aux, i, j, k, J_DIV) = div3l;
+    }
+    return 0;
+}
+
+/* eval_aux_les_cuf (kernels_gpu.F90:10049-10161): over the padded box, from the rho, u,
+ * v, w and Y that eval_velaux left in w_aux: T by Newton, h, p, c and the mixture
+ * transport of eval_aux with the SGS terms added, mu + mu_sgs, k + mu_sgs c_p / Pr_sgs,
+ * rho D_s + mu_sgs / Sc_sgs.  The mixture fraction is not touched. */
+int orc_eval_aux_les(int ns, int nx, int ny, int nz, int ng, int nv, int nv_aux, double *w, double *w_aux,
+                     double p0, double t_min_tab, double dt_tab, double r_univ, const double *rgas,
+                     const double *cv_coeff, const double *cp_coeff, int nsetcv, const double *trange,
+                     int indx_cp_l, int indx_cp_r, double tol_iter_nr, const double *mw, const double *mwinv,
+                     const double *visc_species, const double *lambda_species, const double *diffbin_species,
+                     int num_t_tab, double les_pr, double les_sc, void *stream)
+{
+    (void)stream;
+    if (nv != ns + 4 || nv_aux < J_LES1) return -2;
+    OMP_FOR
+    for (int k = 1 - ng; k <= nz + ng; ++k)
+    for (int j = 1 - ng; j <= ny + ng; ++j)
+    for (int i = 1 - ng; i <= nx + ng; ++i) {
+        double yy[NSMAX];
+        for (int lsp = 1; lsp <= ns; ++lsp) yy[lsp - 1] = W(w_aux, i, j, k, lsp);
+        double rho = W(w_aux, i, j, k, J_R);
+        double ri = 1.0 / rho;
+        double uu = W(w_aux, i, j, k, J_U), vv = W(w_aux, i, j, k, J_V), ww = W(w_aux, i, j, k, J_W);
+        double mu_sgs = W(w_aux, i, j, k, J_LES1);
+        double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        double rhoe = W(w, i, j, k, I_E);
+        double ee = rhoe * ri - qq;
+        double tt = temperature_from_e(ns, ee, W(w_aux, i, j, k, J_T), cv_coeff, nsetcv, trange,
+                                       indx_cp_l, indx_cp_r, tol_iter_nr, yy);
+        double rmixtloc = rgas[0] * yy[0];
+        for (int lsp = 2; lsp <= ns; ++lsp) rmixtloc = rmixtloc + rgas[lsp - 1] * yy[lsp - 1];
+        double cploc = mixture_cp(ns, tt, indx_cp_l, indx_cp_r, cp_coeff, nsetcv, trange, yy);
+        double gamloc = cploc / (cploc - rmixtloc);
+        double pp = rho * tt * rmixtloc;
+        double c = sqrt(gamloc * pp * ri);
+        W(w_aux, i, j, k, J_H) = (rhoe + pp) * ri;
+        W(w_aux, i, j, k, J_T) = tt;
+        W(w_aux, i, j, k, J_P) = pp;
+        W(w_aux, i, j, k, J_C) = c;
+        int itt = (int)((tt - t_min_tab) / dt_tab) + 1;
+        if (itt < 1) itt = 1;
+        if (itt > num_t_tab) itt = num_t_tab;
+        double tloc = t_min_tab + (itt - 1) * dt_tab;
+        double dtt = (tt - tloc) / dt_tab;
+        double mwmixt = r_univ / rmixtloc;
+        double k_cond = 0.0, k_cond2 = 0.0;
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            double dlam = TAB2(lambda_species, itt + 1, lsp) - TAB2(lambda_species, itt, lsp);
+            double klsp = TAB2(lambda_species, itt, lsp) + dlam * dtt;
+            double xlsp = yy[lsp - 1] * mwmixt * mwinv[lsp - 1];
+            k_cond = k_cond + xlsp * klsp;
+            k_cond2 = k_cond2 + xlsp / klsp;
+        }
+        k_cond = 0.5 * (k_cond + 1.0 / k_cond2);
+        double mu = 0.0;
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            double dmu = TAB2(visc_species, itt + 1, lsp) - TAB2(visc_species, itt, lsp);
+            double mulsp = TAB2(visc_species, itt, lsp) + dmu * dtt;
+            double mu_den = 0.0;
+            for (int msp = 1; msp <= ns; ++msp) {
+                dmu = TAB2(visc_species, itt + 1, msp) - TAB2(visc_species, itt, msp);
+                double mumsp = TAB2(visc_species, itt, msp) + dmu * dtt;
+                double t1 = 1.0 + sqrt(mumsp / mulsp) * pow(mw[msp - 1] * mwinv[lsp - 1], 0.25);
+                double phi_lm = 1.0 / sqrt(1.0 + mw[lsp - 1] * mwinv[msp - 1]) * (t1 * t1);
+                mu_den = mu_den + yy[msp - 1] * phi_lm * mwinv[msp - 1];
+            }
+            mu = mu + mulsp * yy[lsp - 1] * mwinv[lsp - 1] / mu_den;
+        }
+        mu = sqrt(8.0) * mu;
+        W(w_aux, i, j, k, J_MU) = mu + mu_sgs;
+        W(w_aux, i, j, k, J_K_COND) = k_cond + mu_sgs * cploc / les_pr;
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            double diff_den = 0.0;
+            for (int msp = 1; msp <= ns; ++msp) {
+                if (msp != lsp) {
+                    double ddiff = TAB3(diffbin_species, itt + 1, lsp, msp) - TAB3(diffbin_species, itt, lsp, msp);
+                    double diff_ij = TAB3(diffbin_species, itt, lsp, msp) + ddiff * dtt;
+                    double xmsp = yy[msp - 1] * mwmixt * mwinv[msp - 1];
+                    diff_den = diff_den + xmsp / diff_ij;
+                }
+            }
+            double rd = diff_den > 1.0e-15 ? (1.0 - yy[lsp - 1]) / diff_den * p0 / pp * rho : 0.0;
+            W(w_aux, i, j, k, J_D_START + lsp) = rd + mu_sgs / les_sc;
+        }
+    }
+    return 0;
+}
+
+/* eval_chem_aux_cuf (kernels_gpu.F90:9676-9877), the output-time chemistry diagnostics on
+ * the interior: the net production rates wdot_s = sum_r (p - r) q_r W_s at the state of
+ * w_aux (rho, T, Y), scaled by the PaSR factor when enable_pasr (which is then stored in
+ * J_GAM_PASR), and the heat release rate -sum_s h298_s wdot_s / W_s into J_HRR (the same
+ * slot: with PaSR the factor is overwritten by the heat release rate, as there). */
+int orc_eval_chem_aux(int ns, int nx, int ny, int nz, int nv_aux, int ng, int nreactions, double *w_aux,
+                      const double *mw, const double *arr_a, const double *arr_b, const double *arr_ea,
+                      const double *tb_eff, const double *falloff_coeffs, const int *reac_ty, const int *isrev,
+                      const double *r_coeffs, const double *p_coeffs, const double *kc_tab, int num_t_tab,
+                      double t_min_tab, double dt_tab, double r_univ, const double *dcsidx, const double *detady,
+                      const double *dzitdz, const double *h298, int enable_pasr, double les_c_yoshi,
+                      double les_c_mix, double les_c_eps, void *stream)
+{
+    (void)stream;
+    if (nv_aux < J_HRR) return -2;
+    OMP_FOR
+    for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+        double conc[NSMAX], yy[NSMAX], wdot[NSMAX];
+        double rho = W(w_aux, i, j, k, J_R);
+        double ri = 1.0 / rho;
+        double tt = W(w_aux, i, j, k, J_T);
+        double Rtti = 1.0 / (r_univ * tt);
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            wdot[lsp - 1] = 0.0;
+            yy[lsp - 1] = W(w_aux, i, j, k, lsp);
+            conc[lsp - 1] = rho * yy[lsp - 1] / mw[lsp - 1];
+        }
+        double tau_k = 0.0;
+        if (enable_pasr > 0)
+            tau_k = pasr_tau_k(rho, W(w_aux, i, j, k, J_LES1), W(w_aux, i, j, k, J_MU), dcsidx[i - 1],
+                               detady[j - 1], dzitdz[k - 1], les_c_yoshi, les_c_mix, les_c_eps);
+        int itt = (int)((tt - t_min_tab) / dt_tab) + 1;
+        if (itt < 1) itt = 1;
+        if (itt > num_t_tab) itt = num_t_tab;
+        double ttleft = (itt - 1) * dt_tab + t_min_tab;
+        double dtt = (tt - ttleft) / dt_tab;
+        for (int lr = 1; lr <= nreactions; ++lr) {
+            double qlr = progress_rate(nreactions, lr, ns, tt, Rtti, itt, dtt, num_t_tab, conc, arr_a, arr_b, arr_ea,
+                                       tb_eff, falloff_coeffs, reac_ty, isrev, r_coeffs, p_coeffs, kc_tab);
+            for (int lsp = 1; lsp <= ns; ++lsp)
+                wdot[lsp - 1] = wdot[lsp - 1] + (RX(p_coeffs, lr, lsp) - RX(r_coeffs, lr, lsp)) * qlr * mw[lsp - 1];
+        }
+        if (enable_pasr > 0) {
+            double tau_cf = 0.0, tau_cs = DBL_MAX;
+            for (int lsp = 0; lsp < ns; ++lsp) {
+                if (fabs(wdot[lsp] * ri) > 1.0e-8) {
+                    double tau_lsp = yy[lsp] / fabs(wdot[lsp] * ri);
+                    if (tau_lsp > tau_cf) tau_cf = tau_lsp;
+                    if (tau_lsp > 0.0 && tau_lsp < tau_cs) tau_cs = tau_lsp;
+                }
+            }
+            double tau_c = sqrt(tau_cs * tau_cf);
+            double gam_pasr = tau_k > DBL_MIN ? tau_c / (tau_c + tau_k) : 1.0;
+            for (int lsp = 0; lsp < ns; ++lsp) wdot[lsp] = gam_pasr * wdot[lsp];
+            W(w_aux, i, j, k, J_GAM_PASR) = gam_pasr;
+        }
+        double hrr = 0.0;
+        for (int lsp = 1; lsp <= ns; ++lsp) {
+            W(w_aux, i, j, k, J_WDOT_START + lsp) = wdot[lsp - 1];
+            hrr = hrr - h298[lsp - 1] * wdot[lsp - 1] / mw[lsp - 1];
+        }
+        W(w_aux, i, j, k, J_HRR) = hrr;
+    }
     return 0;
 }

@@ -5,11 +5,12 @@ multicomponent diffusive fluxes, finite-rate chemistry, stiff per-cell integrati
 copies) runs as hand-written gfx950 HIP kernels in ``libstreams_hip.so`` behind a C-ABI
 (``include/streams_hip.h``).  This package holds the host side around it: mechanism
 tables, seeded inputs, a device-resident field with the reference's RK3 orchestration,
-the batched reactor integrator and the continuous-batching reactor server.
+the time loop of the reference's run with its input-file reader, the batched reactor
+integrator and the continuous-batching reactor server.
 """
 import importlib as _importlib
 
-__all__ = ["mechanism", "capi", "grid", "cases", "solver", "roofline", "reactors", "serving", "build"]
+__all__ = ["mechanism", "capi", "grid", "cases", "solver", "roofline", "reactors", "serving", "build", "driver"]
 
 
 def __getattr__(name):

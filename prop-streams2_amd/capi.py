@@ -28,6 +28,7 @@ _ARGS = {
     "bcswap": "iiiii" "p" "p" "s",
     "bcextr": "iiiiii" "p" "s",
     "bcwall_ad": "iiiiii" "p" "s",
+    "bc_nr_lat": "iiiiiiiii" "ppp" "p" "ii" "ppp" "i" "pp" "s",
     "euler_x": "iiiiiii" "iiii" "pppp" "p" "ii" "ii" "d" "i" "pp" "i" "p" "ii" "p" "ddd" "i" "ii" "p" "s",
     "euler_y": "iiiiiii" "iiii" "pp" "p" "pp" "p" "ii" "ii" "d" "i" "pp" "i" "p" "ii" "p" "ddd" "i" "s",
     "euler_z": "iiiiiii" "iiii" "pp" "p" "pp" "p" "ii" "ii" "d" "i" "pp" "i" "p" "ii" "p" "ddd" "i" "s",
@@ -35,6 +36,7 @@ _ARGS = {
     "visflx_y": "iiiiiii" "p" "p" "pp" "ii" "p" "i" "pp" "d" "s",
     "visflx_z": "iiiiiii" "p" "p" "pp" "ii" "p" "i" "pp" "d" "s",
     "visflx_reduced": "iiiiiii" "p" "p" "pppp" "s",
+    "visflx": "iiiiiii" "pp" "pp" "ppppppppp" "s",
     "compute_chemistry": "iiiiiiii" "ppp" "p" "pppp" "p" "pp" "ppp" "i" "ddd" "ii" "p" "i" "p" "d" "s",
     "rosenbrock": "iiiiiii" "ppp" "p" "i" "pppp" "p" "pp" "ppp" "i" "dddd" "p" "i" "p" "ii" "ddd" "iid" "ii"
                   "ddd" "ppp" "s",
@@ -42,8 +44,17 @@ _ARGS = {
                          "ii" "ddd" "ppp" "s",
     "init_flux": "iiii" "pp" "d" "s",
     "update_flux": "iiii" "pp" "d" "s",
+    "init_flux_simpler": "iiii" "ppp" "d" "s",
+    "sav_flx": "iiii" "pp" "s",
+    "update_simpler_flux": "iiii" "ppp" "d" "s",
     "update_field": "iiiii" "ppp" "s",
     "compute_dt": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
+    "copy_cpu_gpu": "iiiii" "P" "p" "s",
+    "copy_gpu_cpu": "iiiii" "p" "P" "s",
+    "field_interior": "iiiii" "pp" "i" "s",
+    "compute_residual": "iiiii" "p" "d" "P" "p" "s",
+    "compute_rho_t_p_minmax": "iiiiii" "p" "PPPPPP" "p" "s",
+    "find_boom": "iiiii" "p" "d" "p" "ii" "PP" "s",
     "rhs_sweep": "iiiiiiiii" "ppppp" "p" "iiii" "d" "pp" "i" "p" "ii" "p" "ddd" "ii" "d" "i" "d" "pp" "s",
     "rhs_finish": "iiiiiii" "pppppppp" "iii" "ppp" "i" "dddd" "p" "s",
     "reactor_batch": "ii" "pppp" "ppp" "p" "i" "pp" "i" "dddd" "p" "i" "p" "ii" "ii" "d" "pp" "pp" "p" "ii" "s",

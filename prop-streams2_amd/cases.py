@@ -32,7 +32,7 @@ import numpy as np
 
 from .mechanism import Mechanism, R_UNIV, Tables
 
-MECH_DIR = pathlib.Path(__file__).resolve().parents[1] / "tests" / "golden" / "mechanisms"
+MECH_DIR = pathlib.Path(__file__).resolve().parent / "mechanisms"
 H2_AIR_9 = ["H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"]
 
 
@@ -149,6 +149,42 @@ def shock_tube(tables: Tables, nx: int, init_y: Sequence[float], rho0: float, p0
     wp = np.pad(w, [(0, 0), (ng, ng), (ng, ng), (ng, ng)], mode="edge")
     tg = np.where(np.pad(np.broadcast_to(left, (nz, ny, nx)), [(ng, ng)] * 3, mode="edge"), t0, t_t)
     return wp, tg, dict(t0=t0, t_tube=t_t, rho0=rho0, p0=p0)
+
+
+def flow_params(tables: Tables, y: Sequence[float], rho0: float, p0: float, u0: float = 0.0) -> dict:
+    """Reference quantities of set_flow_params (multideal.F90:3415-3539): T0 from the
+    equation of state, the sound speed c0, and u0 replaced by c0 when it is below the Newton
+    tolerance (u0 scales the WENO smoothness indicators and sets eps_sensor = u0^2)."""
+    y = np.asarray(y, dtype=np.float64)
+    rmix = tables.rmix(y)
+    t0 = p0 / (rho0 * rmix)
+    cp = tables.cp_from_T(t0, y)
+    c0 = math.sqrt(cp / (cp - rmix) * rmix * t0)
+    u = c0 if abs(u0) < 1e-9 else u0
+    return dict(t0=t0, c0=c0, u0=u, rho0=rho0, p0=p0, eps_sensor=u * u)
+
+
+def reactor_state(tables: Tables, shape: Sequence[int], ng: int, t0: float, p0: float, y: Sequence[float]):
+    """init_reactor (multideal.F90:5608-5669): every cell at rest at (t0, p0, Y); returns the
+    padded conserved variables (nv, nz+2ng, ny+2ng, nx+2ng) and the temperature guess."""
+    nx, ny, nz = shape
+    y = np.asarray(y, dtype=np.float64)
+    rho = p0 / tables.rmix(y) / t0
+    e = tables.e_from_T(t0, y)
+    ns = tables.n_species
+    w = np.zeros((ns + 4, nz + 2 * ng, ny + 2 * ng, nx + 2 * ng))
+    w[:ns] = (rho * y)[:, None, None, None]
+    w[ns + 3] = rho * e
+    return w, np.full(w.shape[1:], t0)
+
+
+def shock_tube_mechanism() -> Mechanism:
+    """The reactive shock-tube input (9 species H2/O2/.../Ar, 18 reactions)."""
+    return Mechanism(str(mechanism_path("h2_shocktube_9sp.yaml")))
+
+
+# init_Y of the reactive shock-tube example: H2:O2:Ar = 2:1:7 molar
+SHOCK_TUBE_Y = {"H2": 0.01267282566, "O2": 0.10138260528, "Ar": 0.88594456906}
 
 
 def keromnes_mechanism() -> Mechanism:

@@ -46,6 +46,15 @@ def coeff_deriv1_table() -> np.ndarray:
     return t
 
 
+def coeff_deriv2_table() -> np.ndarray:
+    """coeff_deriv2(0:4,4): column lmax holds the order-2*lmax second-derivative
+    coefficients (multideal.F90:688-695)."""
+    t = np.zeros((5, 4), order="F")
+    for lmax in range(1, 5):
+        t[:, lmax - 1] = deriv_coeffs(2 * lmax)[1]
+    return t
+
+
 @dataclasses.dataclass
 class Axis:
     n: int
@@ -55,11 +64,19 @@ class Axis:
     dxi_2: np.ndarray      # dcsidx2
 
 
-def uniform_axis(n: int, length: float, ng: int, periodic: bool, metrics_order: int = 6) -> Axis:
-    """Coordinates x(i) = (i-1)*dx for i = 1-ng .. n+ng and the local metrics."""
-    dx = length / n if periodic else (length / (n - 1) if n > 1 else length)
+def mapped_axis(n: int, length: float, ng: int, periodic: bool, mapping, metrics_order: int = 6) -> Axis:
+    """Axis with node coordinates x(i) = mapping((i-1)*dxi) for i = 1-ng .. n+ng (a
+    stretched grid, as a grid read from file gives), with the local metrics computed from
+    the coordinates by the central differences of compute_metrics (grid.F90:60-133) and
+    set_local_grid (field.F90:300-347): dcsidx = 1/x_xi, dcsidxs = dcsidx^2,
+    dcsidx2 = -x_xixi dcsidxs."""
+    dxi = length / n if periodic else (length / (n - 1) if n > 1 else length)
     idx = np.arange(1 - ng, n + ng + 1)
-    coord = (idx - 1) * dx
+    coord = np.asarray(mapping((idx - 1) * dxi), dtype=np.float64)
+    return _axis_from_coords(n, coord, ng, metrics_order)
+
+
+def _axis_from_coords(n, coord, ng, metrics_order):
     c1, c2 = deriv_coeffs(metrics_order)
     mm = metrics_order // 2
     off = ng - 1                                       # coord position of index i is i + off
@@ -78,6 +95,14 @@ def uniform_axis(n: int, length: float, ng: int, periodic: bool, metrics_order: 
     dxi2 = dxi * dxi
     dxi_2 = -d2xg * dxi2
     return Axis(n=n, coord=coord, dxi=dxi, dxi2=dxi2, dxi_2=dxi_2)
+
+
+def uniform_axis(n: int, length: float, ng: int, periodic: bool, metrics_order: int = 6) -> Axis:
+    """Coordinates x(i) = (i-1)*dx for i = 1-ng .. n+ng and the local metrics."""
+    dx = length / n if periodic else (length / (n - 1) if n > 1 else length)
+    idx = np.arange(1 - ng, n + ng + 1)
+    coord = (idx - 1) * dx
+    return _axis_from_coords(n, coord, ng, metrics_order)
 
 
 def ep_ord_change(nx: int, ny: int, nz: int, bctags, ep_order: int, weno_scheme: int,

@@ -15,6 +15,7 @@ use one that runs the CPU restatement, to compare both sides step by step).
 from __future__ import annotations
 
 import dataclasses
+import pathlib
 from typing import Optional, Sequence
 
 import numpy as np
@@ -88,6 +89,7 @@ class Numerics:
     ros_maxsteps: int = 10000
     ros_maxtry: int = 40
     rosenbrock_version: int = 1      # 1: ROS4 (rosenbrock), 2: Rosenbrock-Krylov
+    operator_splitting: int = 2      # with enable_chemistry == 2: 1 simpler (Krylov only), 2 Strang
 
 
 class Solver:
@@ -96,7 +98,13 @@ class Solver:
     def __init__(self, backend, tables: Tables, nx: int, ny: int, nz: int, ng: int,
                  lengths: Sequence[float], bctags: Sequence[int] = (0, 0, 0, 0, 0, 0),
                  numerics: Optional[Numerics] = None, rho0: float = 1.0, u0: float = 1.0,
-                 p0: Optional[float] = None, eps_sensor: Optional[float] = None):
+                 p0: Optional[float] = None, eps_sensor: Optional[float] = None, mappings=None,
+                 bctags_nr: Sequence[int] = (0, 0, 0, 0, 0, 0), winf: Optional[np.ndarray] = None):
+        """mappings: optional per-axis callables x(xi) giving stretched node coordinates
+        (metrics from the coordinates by the reference's central differences).
+        bctags_nr: the characteristic treatment of each face (xmin_nr .. zmax_nr: 0 none,
+        1 non-reflecting, 2 relaxation to the ghost cell, 3 to winf, 6 reflective wall);
+        winf: the freestream conserved state (nv) that nr_type 3 relaxes to."""
         self.be = backend
         self.tb = tables
         self.ns = tables.n_species
@@ -110,7 +118,9 @@ class Solver:
         self.rho0, self.u0 = rho0, u0
         self.p0 = tables.p0 if p0 is None else p0
         self.eps_sensor = u0 * u0 if eps_sensor is None else eps_sensor
-        self.axes = [gridmod.uniform_axis(n, L, ng, bool(self.periodic[d]))
+        maps = list(mappings) if mappings is not None else [None, None, None]
+        self.axes = [gridmod.uniform_axis(n, L, ng, bool(self.periodic[d])) if maps[d] is None
+                     else gridmod.mapped_axis(n, L, ng, bool(self.periodic[d]), maps[d])
                      for d, (n, L) in enumerate(zip((nx, ny, nz), lengths))]
         be = backend
         px, py, pz = nx + 2 * ng, ny + 2 * ng, nz + 2 * ng
@@ -138,9 +148,11 @@ class Solver:
             self.t_rty = be.asarray(tables.reac_ty, "i4")
             self.t_rev = be.asarray(tables.isRev, "i4")
         self.cd1 = f(gridmod.coeff_deriv1_table())
+        self.cd2 = f(gridmod.coeff_deriv2_table())
         self.t_x, self.t_y, self.t_z = (be.asarray(a.coord) for a in self.axes)
         self.dxi = [be.asarray(a.dxi) for a in self.axes]
         self.dxi2 = [be.asarray(a.dxi2) for a in self.axes]
+        self.dxi_2 = [be.asarray(a.dxi_2) for a in self.axes]
         epo = gridmod.ep_ord_change(nx, ny, nz, self.bctags, self.num.ep_order, self.num.weno_scheme,
                                     self.num.correct_bound_ord)
         self.t_epo = be.asarray(epo.T, "i4")
@@ -149,7 +161,12 @@ class Solver:
         self.t_eoi = be.asarray(np.ones(1))
         self.t_nainsp = be.asarray(np.zeros(self.ns, dtype=np.int32), "i4")
         self.t_beta0 = be.asarray(np.zeros(2))
-        self.eul = (1, nx, 1, ny, 1, nz)   # bc_preproc without non-reflecting faces
+        # bc_preproc (multideal.F90:5671-5718): a face with a characteristic treatment is
+        # left out of the convective update
+        self.bctags_nr = list(bctags_nr)
+        off = [1 if t != 0 else 0 for t in self.bctags_nr]
+        self.eul = (1 + off[0], nx - off[1], 1 + off[2], ny - off[3], 1 + off[4], nz - off[5])
+        self.t_winf = be.asarray(np.asarray(winf, dtype=np.float64)) if winf is not None else None
         self.force_zero_flux = [0] * 6
         self.dt = 0.0
         self.time = 0.0
@@ -160,6 +177,45 @@ class Solver:
         be = self.be
         be.copy_(self.w, be.asarray(w_padded))
         be.copy_(self.w_aux[self.ns + 4], be.asarray(t_guess))
+
+    # ------------------------------------------------------------- restarts
+    def save_field(self, directory, io_type: int = 2, ncoords=(0, 0, 0)):
+        """write_field (io_type 2: rst.bin, field.F90:427-477) or write_field_serial
+        (io_type 1: rst1_XXXX_XXXX_XXXX.bin, :412-425) of the conserved field."""
+        from . import restart
+        be, d = self.be, pathlib.Path(directory)
+        d.mkdir(parents=True, exist_ok=True)
+        if io_type == 2:
+            dense = be.zeros((self.nv, self.nz, self.ny, self.nx))
+            be.call("field_interior", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, dense, 1, be.stream)
+            be.sync()
+            restart.write_rst_bin(d / "rst.bin", be.to_numpy(dense))
+        else:
+            ng = self.ng
+            host = np.empty((self.nz + 2 * ng, self.ny + 2 * ng, self.nx + 2 * ng, self.nv))
+            be.call("copy_gpu_cpu", self.nx, self.ny, self.nz, ng, self.nv, self.w, host, be.stream)
+            restart.write_rst_serial(d / restart.serial_name("rst1", ncoords), host[ng:-ng, ng:-ng, ng:-ng, :])
+
+    def load_field(self, directory, io_type: int = 2, ncoords=(0, 0, 0)):
+        """read_field (rst.bin) or read_field_serial (rst0_XXXX_XXXX_XXXX.bin) into the
+        interior of the conserved field (field.F90:349-410); the ghost layers keep their
+        values until the boundary conditions and the halo exchange renew them."""
+        from . import restart
+        be, d = self.be, pathlib.Path(directory)
+        if io_type == 2:
+            a = restart.read_rst_bin(d / "rst.bin", self.nx, self.ny, self.nz, self.nv)
+            be.call("field_interior", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, be.asarray(a), 0,
+                    be.stream)
+            be.sync()
+        else:
+            ng = self.ng
+            a = restart.read_rst_serial(d / restart.serial_name("rst0", ncoords), self.nx, self.ny, self.nz, self.nv)
+            # the host copy keeps its ghost values, as the reference's host field does across
+            # read_field_serial (which fills 1:nx, 1:ny, 1:nz only) and copy_cpu_gpu
+            host = np.empty((self.nz + 2 * ng, self.ny + 2 * ng, self.nx + 2 * ng, self.nv))
+            be.call("copy_gpu_cpu", self.nx, self.ny, self.nz, ng, self.nv, self.w, host, be.stream)
+            host[ng:-ng, ng:-ng, ng:-ng, :] = a
+            be.call("copy_cpu_gpu", self.nx, self.ny, self.nz, ng, self.nv, host, self.w, be.stream)
 
     # ------------------------------------------------------------- kernels
     def eval_aux(self, box=None):
@@ -234,7 +290,10 @@ class Solver:
                          self.num.visc_order, self.w_aux, self.fl, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2],
                          self.be.stream)
         else:
-            raise NotImplementedError("Laplacian viscous form (conservative_viscous=0)")
+            # the Laplacian form (visflx_cuf): momentum and energy, no species diffusion
+            self.be.call("visflx", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, self.num.visc_order,
+                         self.w_aux, self.fl, self.cd1, self.cd2, *self.dxi, *self.dxi2, *self.dxi_2,
+                         self.be.stream)
 
     def compute_chemistry(self):
         tb = self.tb
@@ -264,8 +323,10 @@ class Solver:
                      tb.indx_cp_r, dt_chem, self.time, self.time + dt_chem, int(fl_sav is not None), fl_sav,
                      n.ros_maxsteps, 0, 0.0, 0.0, 0.0, self.dxi[0], self.dxi[1], self.dxi[2], self.be.stream)
 
-    def update_ghost(self):
-        """Physical boundary conditions of the non-periodic faces (update_ghost, do_swap=0)."""
+    def update_ghost(self, do_swap: int = 0):
+        """update_ghost (multideal_gpu.F90:1165-1238): the physical boundary conditions of
+        the non-periodic faces, then with do_swap = 1 the halo exchange (the RK substeps
+        call it with do_swap = 0)."""
         for ilat, tag in enumerate(self.bctags, start=1):
             if tag == 0:
                 continue
@@ -275,15 +336,51 @@ class Solver:
                 self.be.call("bcwall_ad", self.ns, ilat, self.nx, self.ny, self.nz, self.ng, self.w, self.be.stream)
             else:
                 raise NotImplementedError(f"boundary tag {tag}")
+        if do_swap == 1:
+            self.bcswap()
 
     # -------------------------------------------------------------- RK step
-    def rk_substep(self, istep: int):
-        """One substep of rk_sync (multideal_gpu.F90:562-618)."""
+    def rk_substep(self, istep: int, simpler: bool = False):
+        """One substep of rk_sync (multideal_gpu.F90:562-618); simpler: rk_sync with
+        simpler_splitting = 1, whose stages accumulate fl - fl_sav."""
         n = self.num
         rhodt = WRAY_RHO[istep] * self.dt
         gamdt = WRAY_GAM[istep] * self.dt
         be = self.be
-        be.call("init_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, rhodt, be.stream)
+        if simpler:
+            be.call("init_flux_simpler", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, self.fl_sav, rhodt,
+                    be.stream)
+        else:
+            be.call("init_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, rhodt, be.stream)
+        self._transport_rhs()
+        if n.enable_chemistry == 1:
+            self.compute_chemistry()
+        if simpler:
+            be.call("update_simpler_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, self.fl_sav, gamdt,
+                    be.stream)
+        else:
+            be.call("update_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, gamdt, be.stream)
+        be.call("update_field", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.fln, None, be.stream)
+        self.update_ghost()
+
+    def sav_flx(self):
+        """sav_flx (multideal_gpu.F90:623-692): the right-hand side at the current state,
+        without the field update, kept in fl_sav as the chemistry's transport forcing."""
+        be, n = self.be, self.num
+        if getattr(self, "fl_sav", None) is None:
+            self.fl_sav = be.zeros((self.nv, self.nz, self.ny, self.nx))
+        # the reference's sav_flx passes an unset rhodt: fln is overwritten by the first
+        # init_flux_simpler of the following rk_sync, so its value does not matter
+        be.call("init_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, 0.0, be.stream)
+        self._transport_rhs()
+        if n.enable_chemistry == 1:
+            self.compute_chemistry()
+        self.update_ghost()
+        be.call("sav_flx", self.nx, self.ny, self.nz, self.nv, self.fl, self.fl_sav, be.stream)
+
+    def _transport_rhs(self):
+        """bcswap, compute_aux, sensor exchange, euler_x/y/z and visflx into fl."""
+        n, be = self.num, self.be
         self.bcswap()
         self.compute_aux()
         jduc, jdiv = self.ns + 9, self.ns + 10
@@ -300,17 +397,26 @@ class Solver:
             self.visflx(2)
         else:
             self.visflx(0)
-        if n.enable_chemistry == 1:
-            self.compute_chemistry()
-        be.call("update_flux", self.nx, self.ny, self.nz, self.nv, self.fl, self.fln, gamdt, be.stream)
-        be.call("update_field", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.fln, None, be.stream)
-        self.update_ghost()
+        self.bc_nr()
+
+    def bc_nr(self):
+        """bc_nr (multideal_gpu.F90:1116-1163): the characteristic treatment of the faces
+        that have one."""
+        tb = self.tb
+        for ilat, nr_type in enumerate(self.bctags_nr, start=1):
+            if nr_type == 0:
+                continue
+            d = (ilat - 1) // 2
+            self.be.call("bc_nr_lat", d + 1, (ilat - 1) % 2 + 1, nr_type, self.ns, self.nx, self.ny, self.nz, self.ng,
+                         self.nv, self.w_aux, self.w, self.fl, self.dxi[d], tb.indx_cp_l, tb.indx_cp_r, self.t_rgas,
+                         self.t_cv, self.t_cp, tb.nsetcv, self.t_trange, self.t_winf, self.be.stream)
 
     def fused_supported(self) -> bool:
         """The fused substep covers full-range sweeps (no non-reflecting offsets), the
-        conservative viscous form and explicit or no chemistry."""
+        conservative viscous form; explicit chemistry is folded into the tail, implicit
+        chemistry (enable_chemistry = 2) runs around the substeps."""
         return (self.eul == (1, self.nx, 1, self.ny, 1, self.nz) and self.num.conservative_viscous == 1
-                and self.num.enable_chemistry in (0, 1) and self.be.has("rhs_sweep") and self.be.has("rhs_finish"))
+                and self.be.has("rhs_sweep") and self.be.has("rhs_finish"))
 
     def rk_substep_fused(self, istep: int):
         """One RK substep in five passes: halo + aux, sensor, fused x/y/z sweeps (with
@@ -346,22 +452,35 @@ class Solver:
                 *kin, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV, gamdt, None, be.stream)
         self.update_ghost()
 
-    def rk_sync(self, fused: Optional[bool] = None):
+    def rk_sync(self, fused: Optional[bool] = None, simpler: bool = False):
+        """rk_sync (multideal_gpu.F90:536-621).  The simpler-splitting stages run routine
+        by routine (the fused passes fold update_flux, not its simpler variant)."""
         if fused is None:
             fused = self.fused_supported()
         for istep in range(3):
-            if fused:
+            if fused and not simpler:
                 self.rk_substep_fused(istep)
             else:
-                self.rk_substep(istep)
+                self.rk_substep(istep, simpler=simpler)
 
     def advance(self, fused: Optional[bool] = None):
         """advance_solution (multideal_gpu.F90:1853-1931) without LES / energy deposition."""
-        if self.num.enable_chemistry == 2:
+        n = self.num
+        if n.enable_chemistry == 2 and n.operator_splitting == 1:
+            if n.rosenbrock_version != 2:
+                raise ValueError("simpler splitting needs the Rosenbrock-Krylov integrator (rosenbrock_version 2)")
+            dt = self.dt
+            self.dt = 0.5 * dt
+            self.sav_flx()
+            self.compute_aux(central=1, ghost=0)
+            self.rosenbrock_krylov(dt, fl_sav=self.fl_sav)
+            self.rk_sync(fused=False, simpler=True)
+            self.dt = dt
+        elif n.enable_chemistry == 2:
             dt_chem = 0.5 * self.dt
-            chem = self.rosenbrock if self.num.rosenbrock_version == 1 else self.rosenbrock_krylov
+            chem = self.rosenbrock if n.rosenbrock_version == 1 else self.rosenbrock_krylov
             chem(dt_chem)
-            self.rk_sync(fused=False)
+            self.rk_sync(fused=fused)
             self.compute_aux(central=1, ghost=0)
             chem(dt_chem)
         else:

@@ -51,7 +51,7 @@ def test_kinetics_pack_restates_the_dense_tables(pkg):
     kinetics tables (stoichiometry, net coefficients times W_s, rates, efficiencies)."""
     import numpy as np
     for mech in (pkg.cases.h2_air_mechanism(),
-                 pkg.mechanism.Mechanism(str(ROOT / "tests/golden/mechanisms/keromnes2013_14sp.yaml"))):
+                 pkg.mechanism.Mechanism(str(pkg.cases.mechanism_path("keromnes2013_14sp.yaml")))):
         tb = mech.tables(transport=False)
         ns, nr = tb.n_species, tb.nreactions
         pack = pkg.capi.kinetics_pack(tb).reshape(nr, 80)

@@ -196,10 +196,94 @@ def test_euler_boundary_order_reduction(pkg, h2_tables, hip_backend, cpu_backend
     assert err.max() < TOL, err
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_visflx(pkg, h2_tables, hip_backend, cpu_backend, mode):
     err = _flux_case(pkg, h2_tables, hip_backend, cpu_backend, None, lambda s: s.visflx(mode), perturb_y=0.3)
     assert err.max() < TOL, err
+
+
+def test_laplacian_viscous_substeps_on_a_stretched_grid(pkg, h2_tables, hip_backend, cpu_backend):
+    """conservative_viscous = 0 (visflx_cuf in rk_sync) on a grid stretched in x and z, so
+    that the second-derivative metrics are not zero: three substeps routine by routine."""
+    L = 2 * np.pi * 1e-3
+    maps = [lambda xi: xi + 0.2 * L / (2 * np.pi) * np.sin(2 * np.pi * xi / L), None,
+            lambda xi: xi - 0.15 * L / (2 * np.pi) * np.sin(2 * np.pi * xi / L)]
+    num = pkg.solver.Numerics(conservative_viscous=0)
+    w, tg, info = box_state(pkg, h2_tables, SHAPE, seed=13, temperature=1400.0, perturb_y=0.2)
+    out = []
+    for be in (hip_backend, cpu_backend):
+        s = pkg.solver.Solver(be, h2_tables, *SHAPE, NG, [info["length"]] * 3, numerics=num, rho0=info["rho0"],
+                              u0=info["u_rms"], mappings=maps)
+        s.set_state(w, tg)
+        s.compute_aux()
+        out.append(s)
+    g, c = out
+    c.compute_dt(0.5)
+    g.dt = c.dt
+    assert not g.fused_supported()
+    g.rk_substep(0)
+    c.rk_substep(0)
+    g.be.sync()
+    err_fl = rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+    for q in (1, 2):
+        g.rk_substep(q)
+        c.rk_substep(q)
+    g.be.sync()
+    err_w = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
+    assert err_fl.max() < TOL, err_fl
+    assert err_w.max() < 1e-13, err_w
+
+
+@pytest.mark.parametrize("nr_type", [1, 2, 3, 6])
+def test_bc_nr_lat_all_faces(pkg, h2_tables, hip_backend, cpu_backend, nr_type):
+    """bc_nr_lat on all six faces of a non-periodic box (one call per face, as bc_nr does)."""
+    w, tg, info = box_state(pkg, h2_tables, SHAPE, seed=31, mach_t=0.5, perturb_y=0.2)
+    winf = w[:, NG + 2, NG + 3, NG + 4].copy() * 1.01
+    out = []
+    for be in (hip_backend, cpu_backend):
+        s = pkg.solver.Solver(be, h2_tables, *SHAPE, NG, [info["length"]] * 3, bctags=(2,) * 6, bctags_nr=(nr_type,) * 6,
+                              winf=winf, rho0=info["rho0"], u0=info["u_rms"])
+        s.set_state(w, tg)
+        out.append(s)
+    g, c = out
+    c.compute_aux()
+    sync_aux(g, c)
+    for s in (g, c):
+        s.be.copy_(s.fl, s.be.asarray(np.zeros(c.fl.shape)))
+        s.bc_nr()
+    g.be.sync()
+    err = rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+    assert np.abs(c.fl).max() > 0.0
+    assert err.max() < TOL, err
+
+
+def test_substeps_with_characteristic_faces(pkg, h2_tables, hip_backend, cpu_backend):
+    """rk_sync routine by routine with a non-reflecting xmin face, a freestream-relaxation
+    xmax face and a reflective-wall zmax face: the face cells leave the convective update
+    (bc_preproc) and take bc_nr's term instead."""
+    w, tg, info = box_state(pkg, h2_tables, SHAPE, seed=33, mach_t=0.4, perturb_y=0.2, temperature=1300.0)
+    winf = w[:, NG, NG, NG].copy()
+    out = []
+    for be in (hip_backend, cpu_backend):
+        s = pkg.solver.Solver(be, h2_tables, *SHAPE, NG, [info["length"]] * 3, bctags=(2, 2, 0, 0, 2, 2),
+                              bctags_nr=(1, 3, 0, 0, 0, 6), winf=winf, rho0=info["rho0"], u0=info["u_rms"])
+        s.set_state(w, tg)
+        s.compute_aux()
+        out.append(s)
+    g, c = out
+    assert g.eul == (2, SHAPE[0] - 1, 1, SHAPE[1], 1, SHAPE[2] - 1) and not g.fused_supported()
+    c.compute_dt(0.5)
+    g.dt = c.dt
+    for q in range(3):
+        g.rk_substep(q)
+        c.rk_substep(q)
+        if q == 0:
+            g.be.sync()
+            err_fl = rel_maxnorm(g.be.to_numpy(g.fl), c.fl)
+    g.be.sync()
+    err_w = rel_maxnorm(interior(g.be.to_numpy(g.w), NG), interior(c.w, NG))
+    assert err_fl.max() < TOL, err_fl
+    assert err_w.max() < 1e-13, err_w
 
 
 def test_chemistry(pkg, h2_tables, hip_backend, cpu_backend):
@@ -305,6 +389,64 @@ def test_rosenbrock_krylov_cells(pkg, h2_tables, hip_backend, cpu_backend):
             c.rosenbrock_krylov(2e-6, f)
             bound = max(bound, 10.0 * rel_maxnorm(interior(c.w, NG), interior(ref, NG)).max())
         assert err.max() < bound, (forcing, bound, err)
+
+
+def _advance_pair(pkg, tables, hip_backend, cpu_backend, numerics, shape, steps, seed=17, **kw):
+    """`steps` calls of advance() on both sides from the same state, plus the oracle's own
+    result from that state perturbed by 1e-15 (relative), all as interior (w, T)."""
+    w, tg, info = box_state(pkg, tables, shape, seed=seed, **kw)
+    probe = pkg.solver.Solver(cpu_backend, tables, *shape, NG, [info["length"]] * 3, numerics=numerics,
+                              rho0=info["rho0"], u0=info["u_rms"])
+    probe.set_state(w, tg)
+    probe.compute_aux()
+    dt = probe.compute_dt(0.5)
+
+    def run(be, w0):
+        s = pkg.solver.Solver(be, tables, *shape, NG, [info["length"]] * 3, numerics=numerics,
+                              rho0=info["rho0"], u0=info["u_rms"])
+        s.set_state(w0, tg)
+        s.compute_aux()
+        s.dt = dt
+        for _ in range(steps):
+            s.advance()
+        s.compute_aux(central=1, ghost=0)
+        s.be.sync()
+        ns = tables.n_species
+        return (interior(s.be.to_numpy(s.w), NG).copy(), interior(s.be.to_numpy(s.w_aux), NG)[ns + 4].copy())
+
+    wp = w * (1.0 + 1e-15 * np.random.default_rng(2).standard_normal(w.shape))
+    return run(hip_backend, w), run(cpu_backend, w), run(cpu_backend, wp)
+
+
+def test_simpler_splitting_advance(pkg, h2_tables, hip_backend, cpu_backend):
+    """advance_solution with operator_splitting = 1 (multideal_gpu.F90:1875-1893): sav_flx,
+    Rosenbrock-Krylov chemistry over dt with the saved transport RHS as forcing, then
+    rk_sync over dt/2 with init_flux_simpler / update_simpler_flux.  The forcing makes the
+    Krylov accept/reject and subspace-size decisions sensitive to rounding (as in
+    test_rosenbrock_krylov_cells), so species and T are held to ten times the oracle's own
+    response to a 1e-15 input perturbation, never below 1e-9."""
+    num = pkg.solver.Numerics(enable_chemistry=2, rosenbrock_version=2, operator_splitting=1)
+    (gw, gt), (cw, ct), (pw, pt) = _advance_pair(pkg, h2_tables, hip_backend, cpu_backend, num, (8, 6, 5), 2,
+                                                 temperature=1500.0, perturb_y=0.3)
+    bound_w = max(1e-9, 10.0 * rel_maxnorm(pw, cw).max())
+    bound_t = max(1e-9, 10.0 * np.abs(pt - ct).max() / ct.max())
+    err = rel_maxnorm(gw, cw)
+    assert err.max() < bound_w, (bound_w, err)
+    assert np.abs(gt - ct).max() / ct.max() < bound_t
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_strang_advance_on_the_fused_substeps(pkg, h2_tables, hip_backend, cpu_backend, version):
+    """advance_solution with operator_splitting = 2 (Strang, multideal_gpu.F90:1894-1917):
+    implicit chemistry over dt/2, the fused RK substeps on the GPU (routine by routine on
+    the oracle), implicit chemistry over dt/2; ROS4 and Rosenbrock-Krylov.  Species and T
+    after three steps within 1e-9."""
+    num = pkg.solver.Numerics(enable_chemistry=2, rosenbrock_version=version, operator_splitting=2)
+    (gw, gt), (cw, ct), _ = _advance_pair(pkg, h2_tables, hip_backend, cpu_backend, num, (8, 6, 5), 3,
+                                          temperature=1500.0, perturb_y=0.3)
+    err = rel_maxnorm(gw, cw)
+    assert err.max() < 1e-9, err
+    assert np.abs(gt - ct).max() / ct.max() < 1e-9
 
 
 # ---------------------------------------------------------------- fused substep stages

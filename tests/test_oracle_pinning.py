@@ -36,7 +36,7 @@ def be():
 
 
 def _tables(pkg, name, **kw):
-    return pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / name)).tables(p0=101325.0, **kw)
+    return pkg.mechanism.Mechanism(str(pkg.cases.mechanism_path(name))).tables(p0=101325.0, **kw)
 
 
 def _batch_history(pkg, be, mech):
@@ -241,8 +241,8 @@ def test_kinetics_relax_to_gibbs_equilibrium(pkg, be):
 
 
 def test_chemical_source_conserves_mass_and_elements(pkg, be):
-    tb = pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / "keromnes2013_14sp.yaml")).tables(p0=101325.0)
-    mech = pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / "keromnes2013_14sp.yaml"))
+    tb = pkg.mechanism.Mechanism(str(pkg.cases.mechanism_path("keromnes2013_14sp.yaml"))).tables(p0=101325.0)
+    mech = pkg.mechanism.Mechanism(str(pkg.cases.mechanism_path("keromnes2013_14sp.yaml")))
     ns = tb.n_species
     rng = np.random.default_rng(17)
     npts = 40
@@ -533,7 +533,7 @@ def test_weno_shock_tube_against_exact_riemann(pkg, be, flux_splitting):
     branch on every face (sensor threshold below the sensor's range), Roe characteristic
     or Rusanov splitting, extrapolation at both ends: the density matches the exact
     Riemann solution (L1 error of a few percent of the jump, shock and contact positions)."""
-    mech = pkg.mechanism.Mechanism(str(GOLD / "mechanisms" / "h2_premix_11sp.yaml"), species=["Ar"], reactions=False)
+    mech = pkg.mechanism.Mechanism(str(pkg.cases.mechanism_path("h2_premix_11sp.yaml")), species=["Ar"], reactions=False)
     tb = mech.tables()
     n, length = 200, 1.0
     num = pkg.solver.Numerics(enable_chemistry=0, sensor_threshold=-1.0, weno_scheme=3, weno_version=1,
