@@ -10,7 +10,12 @@ Prints one JSON line with the roofline of the dominant entry point (timed live w
 events on the launch stream) and the CPU restatement's rate on a bounded sample of the
 same workload (rank 0, N = 1).
 
+Besides the 256^3 line, the default run measures the other half of the metric: the
+batched stiff chemistry (2^21 reactors, mixed precision, with the FP64 integrators on a
+2^18 sample) and the reactor server's throughput and p50/p99 latency on a Poisson stream.
+
     python bench.py [--gpus 1] [--steps 4] [--warmup 1] [--n 256] [--no-cpu]
+                    [--reactors 2097152] [--serving 0.25]
 """
 from __future__ import annotations
 
@@ -25,7 +30,7 @@ import time
 import numpy as np
 
 ROOT = pathlib.Path(__file__).resolve().parent
-METRIC = "cell-updates/s (RHS+chem) at 256^3"
+METRIC = "cell-updates/s (RHS+chem) at {n}^3; reactor-solves/s + p99 latency at 1e-6"
 
 
 def _pkg():
@@ -126,7 +131,7 @@ def cpu_baseline(pkg, tables, n_sample: int, steps: int):
     times.sort()
     med = times[len(times) // 2]
     rate = 3 * n_sample ** 3 / med
-    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{n_sample}^3 box of the same workload, {steps} RK3 steps (3 substeps each), "
                       f"median step; oracle/oracle.c FP64 restatement, OpenMP over z-slabs"}
 
@@ -147,54 +152,91 @@ def copy_probe(gib: float):
     print(json.dumps({"copy_bytes_each_way": 8 * n, "ms": ms, "GBps_read_plus_write": 16 * n / ms / 1e6}))
 
 
-def reactor_bench(pkg, n: int, reps: int = 3):
+def reactor_bench(pkg, n: int, n_fp64: int = 262144, reps: int = 1):
     """Batched constant-volume reactors (T log-uniform 800-2500 K, p 0.5-20 atm, Dirichlet
-    Y, horizon 1e-6 s): reactor-solves/s of the FP64 Rosenbrock batch and of its
-    mixed-precision variant, and the relative differences in final T between the two (the
-    final state inverted for T on the host)."""
+    Y, horizon 1e-6 s, tolerance 1e-8): reactor-solves/s of the mixed-precision species-block
+    integrator on all n reactors, and of the plain FP64 integrator (the reference's
+    algorithm and arithmetic) and of the FP64 species-block integrator on the first n_fp64
+    of them (the per-reactor cost distribution is the same: the reactors are independent
+    draws); final-T differences of the mixed and the FP64 species-block forms against the
+    plain FP64 one on those n_fp64 reactors (the final states inverted for T on the host);
+    the reactors whose state leaves the tables' temperature range are counted and listed."""
     import torch
     tables = pkg.cases.h2_air_mechanism().tables()
     w, tg, dttry, tend, info = pkg.reactors.reactor_inputs(tables, n)
     be = pkg.solver.HipBackend("cuda:0")
     G = pkg.reactors.ReactorBatchGPU(be, tables, tol=1e-8)
-    d_dt, d_end = be.asarray(dttry), be.asarray(tend)
-    na, nr = be.zeros((n,), "i4"), be.zeros((n,), "i4")
-    out = {"reactors": n, "horizon_s": float(tend[0]), "tol": 1e-8}
+    n_fp64 = min(n, n_fp64)
+    out = {"reactors": n, "reactors_fp64_sample": n_fp64, "horizon_s": float(tend[0]), "tol": 1e-8,
+           "workload": "2^21 independent constant-volume reactors, 9-species H2-air, T log-uniform 800-2500 K, "
+                       "p log-uniform 0.5-20 atm, Dirichlet(1) mass fractions, dt = 1e-6 s"
+                       if n == 1 << 21 else f"{n} reactors of the batched-chemistry workload"}
     finals = {}
-    for mixed in (False, True):
+    for key, mode, nb in (("mixed", 1, n), ("fp64", 0, n_fp64), ("fp64_species_block", 2, n_fp64)):
+        print(f"[bench] reactors: {key} on {nb}", file=sys.stderr, flush=True)
+        d_dt, d_end = be.asarray(dttry[:nb]), be.asarray(tend[:nb])
+        na, nr = be.zeros((nb,), "i4"), be.zeros((nb,), "i4")
         times = []
         for rep in range(reps + 1):
-            wd, td = be.asarray(w), be.asarray(tg)
+            wd, td = be.asarray(np.ascontiguousarray(w[:, :nb])), be.asarray(tg[:nb])
             torch.cuda.synchronize()
+            st = torch.cuda.current_stream()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            G.integrate(wd, td, d_dt, d_end, na, nr, mixed=mixed)
-            e1.record()
+            e0.record(st)
+            G.integrate(wd, td, d_dt, d_end, na, nr, mixed=mode)
+            e1.record(st)
             torch.cuda.synchronize()
-            if rep > 0:
+            if rep > 0 or reps == 0:
                 times.append(e0.elapsed_time(e1) / 1e3)
-        times.sort()
-        t = times[len(times) // 2]
+        t = sorted(times)[len(times) // 2]
         nacc = be.to_numpy(na)
-        key = "mixed" if mixed else "fp64"
         wf = be.to_numpy(wd)
-        finals[key] = wf
-        out[key] = {"seconds": t, "reactor_solves_per_s": n / t, "steps_accepted_mean": float(nacc.mean()),
+        finals[key] = wf[:, :n_fp64]
+        bad = np.nonzero(~np.isfinite(wf).all(axis=0))[0]
+        out[key] = {"seconds": t, "reactor_solves_per_s": nb / t, "steps_accepted_mean": float(nacc.mean()),
                     "steps_accepted_max": int(nacc.max()), "steps_rejected_total": int(be.to_numpy(nr).sum()),
-                    "nonfinite_reactors": int((~np.isfinite(wf).all(axis=0)).sum())}
-    out["mixed_speedup"] = out["mixed"]["reactor_solves_per_s"] / out["fp64"]["reactor_solves_per_s"]
-    a, b = finals["mixed"], finals["fp64"]
-    fin = np.isfinite(a).all(axis=0) & np.isfinite(b).all(axis=0)
-    scale = np.abs(b[:, fin]).max(axis=1)
-    dif = np.abs(a[:, fin] - b[:, fin]).max(axis=1)
-    out["mixed_vs_fp64_max_rel_diff_per_variable"] = float((dif[scale > 0] / scale[scale > 0]).max())
-    ta = pkg.reactors.final_temperatures(tables, a, tg)
-    tb_ = pkg.reactors.final_temperatures(tables, b, tg)
-    okT = np.isfinite(ta) & np.isfinite(tb_)
-    dT = np.abs(ta[okT] - tb_[okT]) / tb_[okT]
-    out["mixed_vs_fp64_final_T_rel_diff"] = {"p50": float(np.median(dT)), "p99": float(np.quantile(dT, 0.99)),
-                                             "max": float(dT.max()), "fraction_above_1e-6": float((dT > 1e-6).mean())}
+                    "nonfinite_reactors": int(bad.size), "nonfinite_first": bad[:8].tolist()}
+    out["mixed_speedup_vs_fp64"] = out["mixed"]["reactor_solves_per_s"] / out["fp64"]["reactor_solves_per_s"]
+    out["mixed_speedup_vs_fp64_species_block"] = (out["mixed"]["reactor_solves_per_s"]
+                                                  / out["fp64_species_block"]["reactor_solves_per_s"])
+    t0 = pkg.reactors.final_temperatures(tables, finals["fp64"], tg[:n_fp64])
+    for key in ("mixed", "fp64_species_block"):
+        tk = pkg.reactors.final_temperatures(tables, finals[key], tg[:n_fp64])
+        ok = np.isfinite(tk) & np.isfinite(t0)
+        dT = np.abs(tk[ok] - t0[ok]) / t0[ok]
+        out[f"{key}_vs_fp64_final_T_rel_diff"] = {
+            "p50": float(np.median(dT)), "p99": float(np.quantile(dT, 0.99)), "max": float(dT.max()),
+            "fraction_above_1e-6": float((dT > 1e-6).mean())}
     return out
+
+
+def cpu_reactor_baseline(pkg, n: int = 2048):
+    """The oracle's plain FP64 Rosenbrock (OpenMP over reactors) on the first n reactors of
+    the batched-chemistry workload: reactor-solves/s on the host cores."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_backend
+    import reactor_util
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    tables = pkg.cases.h2_air_mechanism().tables()
+    w, tg, dttry, tend, info = pkg.reactors.reactor_inputs(tables, n)
+    R = reactor_util.ReactorBatch(oracle_backend.load_oracle("liboracle_omp.so"), tables, tol=1e-8)
+    wc = np.ascontiguousarray(w)
+    t0 = time.perf_counter()
+    R.integrate(wc, tg.copy(), dttry, tend)
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "reactor-solves/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} reactors of the batched-chemistry workload; oracle/oracle.c plain FP64 "
+                      f"Rosenbrock, OpenMP over reactors"}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def serving_bench(pkg, duration: float, rate: float = 200.0, budget: int = 200):
@@ -203,7 +245,7 @@ def serving_bench(pkg, duration: float, rate: float = 200.0, budget: int = 200):
     (Keromnes 14 species), T0 900-1800 K, phi 0.3-3 (log-uniform), horizon 1e-5..1e-2 s
     (log-uniform), served with continuous batching; throughput and latency percentiles."""
     mechs = {"h2": pkg.cases.h2_air_mechanism().tables(),
-             "syngas": pkg.mechanism.Mechanism(str(ROOT / "tests/golden/mechanisms/keromnes2013_14sp.yaml")).tables()}
+             "syngas": pkg.cases.keromnes_mechanism().tables()}
     fuels = {"h2": {"H2": 1.0}, "syngas": {"H2": 0.5, "CO": 0.5}}
     reqs = pkg.serving.poisson_requests(mechs, rate=rate, duration=duration, seed=2024, fuels=fuels)
     be = pkg.solver.HipBackend("cuda:0")
@@ -259,10 +301,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--reactors", type=int, default=0,
-                    help="also time the batched chemistry on this many reactors (e.g. 2097152)")
-    ap.add_argument("--serving", type=float, default=0.0,
-                    help="also run the reactor server on a Poisson stream of this many seconds")
+    ap.add_argument("--reactors", type=int, default=1 << 21,
+                    help="batched chemistry on this many reactors (0: skip)")
+    ap.add_argument("--reactors-fp64", type=int, default=1 << 18,
+                    help="reactors of the same workload timed with the FP64 integrators")
+    ap.add_argument("--serving", type=float, default=0.25,
+                    help="reactor server on a Poisson stream of this many seconds (0: skip)")
     ap.add_argument("--c4", action="store_true",
                     help="also time the 512x256x128 14-species syngas mixing layer with ~10 %% WENO faces")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -321,7 +365,7 @@ def main():
     survey_bytes = rf.substep_bytes_survey(tables.n_species, cells, n)
     breakdown = {k: round(v[0] / v[1], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
     out = {
-        "metric": METRIC,
+        "metric": METRIC.format(n=n),
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -349,7 +393,7 @@ def main():
     }
     if args.reactors > 0:
         print(f"[bench] batched chemistry, {args.reactors} reactors", file=sys.stderr, flush=True)
-        out["batched_chemistry"] = reactor_bench(pkg, args.reactors)
+        out["batched_chemistry"] = reactor_bench(pkg, args.reactors, args.reactors_fp64)
     if args.serving > 0:
         print(f"[bench] reactor server, {args.serving} s stream", file=sys.stderr, flush=True)
         out["serving"] = serving_bench(pkg, args.serving)
@@ -358,6 +402,8 @@ def main():
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pkg, tables, args.cpu_n, args.cpu_steps)
+        if args.reactors > 0:
+            out["cpu_baseline_reactors"] = cpu_reactor_baseline(pkg)
     print(json.dumps(out))
 
 

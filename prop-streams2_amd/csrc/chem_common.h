@@ -133,12 +133,16 @@ __device__ void ros_rhs(const Kin &K, const Thermo &th, double tol_iter_nr, doub
   }
 }
 
-// ludcmp: Crout LU with implicit partial pivoting, a is n x n column-major
+// ludcmp: Crout LU with implicit partial pivoting, a is n x n column-major.  A row whose
+// largest magnitude is below tiny (or NaN) ends the factorisation early, as in the
+// reference, which then leaves indx unset; here indx starts as the identity permutation so
+// that lubksb never indexes outside its vector (the result is then not a solve, as there).
 template <int N, class T = double>
 __device__ void ludcmp(T *a, int *indx) {
   const T tiny = (T)1.5e-16;
   T vv[N];
 #define A_(r, c) a[(r) + N * (c)]
+  for (int i = 0; i < N; ++i) indx[i] = i;
   for (int i = 0; i < N; ++i) {
     T amax = 0;
     for (int j = 0; j < N; ++j) amax = fabs(A_(i, j)) > amax ? fabs(A_(i, j)) : amax;

@@ -53,13 +53,63 @@ __device__ inline double conc_power(double c, double ex) {
   return exp(ex * log(c));
 }
 
+// The record's (reactant, product) exponents of species s (0 when s does not take part):
+// the values of r_coeffs(lr,s) and p_coeffs(lr,s).
+template <int S>
+__device__ inline void record_coeffs(kdouble *R, kint *I, int nre, int npr, double &rr, double &pp) {
+  rr = 0.0;
+  pp = 0.0;
+#pragma unroll
+  for (int e = 0; e < KSIDE; ++e) {
+    if (e < nre && I[KI_RE + e] == S) rr = R[KD_REX + e];
+    if (e < npr && I[KI_PR + e] == S) pp = R[KD_PEX + e];
+  }
+}
+
+template <int S, int NS>
+struct RefSpecies {
+  // q1 = prod c^r, q2 = prod c^p in ascending species order with the reference's
+  // exponent cases (1: multiply, other non-zero: pow)
+  __device__ static inline void products(kdouble *R, kint *I, int nre, int npr, const double *conc, double &q1,
+                                         double &q2) {
+    double rr, pp;
+    record_coeffs<S>(R, I, nre, npr, rr, pp);
+    if (rr == 1.0) q1 = q1 * conc[S];
+    else if (rr != 0.0) q1 = q1 * pow(conc[S], rr);
+    if (pp == 1.0) q2 = q2 * conc[S];
+    else if (pp != 0.0) q2 = q2 * pow(conc[S], pp);
+    RefSpecies<S + 1, NS>::products(R, I, nre, npr, conc, q1, q2);
+  }
+  // acc_s -= ((p - r) q) W_s for the species of the reaction
+  __device__ static inline void accumulate(kdouble *R, kint *I, int nre, int npr, double q, const double *mw,
+                                           double *acc) {
+    double rr, pp;
+    record_coeffs<S>(R, I, nre, npr, rr, pp);
+    if (rr != 0.0 || pp != 0.0) acc[S] = acc[S] - ((pp - rr) * q) * mw[S];
+    RefSpecies<S + 1, NS>::accumulate(R, I, nre, npr, q, mw, acc);
+  }
+};
+template <int NS>
+struct RefSpecies<NS, NS> {
+  __device__ static inline void products(kdouble *, kint *, int, int, const double *, double &, double &) {}
+  __device__ static inline void accumulate(kdouble *, kint *, int, int, double, const double *, double *) {}
+};
+
 // Species source -omega_s W_s accumulated into acc[0..NS) for one cell at temperature tt
 // with molar concentrations conc (compute_chemistry_cuf's reaction loop,
-// kernels_gpu.F90:9551-9649).  Rate constants use A exp(b ln T - Ea/RT), log10(x) =
-// ln(x)/ln(10) and 10^x = exp(x ln 10); these identities change results in the last
-// digits only.
-template <int NS>
-__device__ inline void chemistry_source_packed(const KinPacked &K, double tt, const double *conc, double *acc) {
+// kernels_gpu.F90:9551-9649).
+//  * REF = false (the fused RHS and the fast reactor paths): rate constants through
+//    A exp(b ln T - Ea/RT), log10(x) = ln(x)/ln(10), 10^x = exp(x ln 10), c^2 = c*c, and the
+//    net coefficient (p - r) W_s taken from the record; these identities change results in
+//    the last digits only.
+//  * REF = true (the plain FP64 reactor integrator, the yardstick of the others): the
+//    reference's own forms and operation order (rosenbrock_rhs, :9249-9339): A T^b exp(-Ea/RT)
+//    with the b = 0 and Ea = 0 cases skipped, log10, pow(10, .), pow for exponents other
+//    than 1, products over the species in ascending order, acc -= ((p - r) q) W_s with
+//    mw = W_s.
+template <int NS, bool REF = false>
+__device__ inline void chemistry_source_packed(const KinPacked &K, double tt, const double *conc, double *acc,
+                                               const double *mw = nullptr) {
   static_assert(NS <= KTB, "third-body efficiencies of a record hold at most KTB species");
   constexpr double LN10 = 2.302585092994045684;
   constexpr double ILN10 = 0.434294481903251827651;
@@ -76,7 +126,48 @@ __device__ inline void chemistry_source_packed(const KinPacked &K, double tt, co
     kint *I = (kint *)R;
     const int ty = I[KI_TY];
     double kf, tbv = 1.0;
-    if (ty <= 1) {
+    if (REF) {
+      if (ty <= 1) {
+        kf = R[KD_A];
+        if (R[KD_A + 1] != 0.0) kf = kf * pow(tt, R[KD_A + 1]);
+        if (R[KD_A + 2] != 0.0) kf = kf * exp(-R[KD_A + 2] * Rtti);
+        if (ty == 1) {
+          tbv = 0.0;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) tbv = tbv + R[KD_TB + s] * conc[s];
+        }
+      } else {
+        const double k0 = R[KD_A] * pow(tt, R[KD_A + 1]) * exp(-R[KD_A + 2] * Rtti);
+        const double kinf = R[KD_AINF] * pow(tt, R[KD_AINF + 1]) * exp(-R[KD_AINF + 2] * Rtti);
+        double tb = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) tb = tb + R[KD_TB + s] * conc[s];
+        const double rpres = k0 * tb / kinf;
+        if (ty == 2) {
+          kf = kinf * (rpres / (1.0 + rpres));
+        } else if (ty == 3) {
+          if (rpres < 1e-30) {
+            kf = k0 * tb;
+          } else {
+            double fcent = (1.0 - R[KD_FC]) * exp(-tt / R[KD_FC + 1]) + R[KD_FC] * exp(-tt / R[KD_FC + 2]);
+            if (I[KI_T2]) fcent = fcent + exp(-R[KD_FC + 3] / tt);
+            fcent = log10(fcent);
+            const double coff = -0.4 - 0.67 * fcent;
+            const double noff = 0.75 - 1.27 * fcent;
+            const double doff = 0.14;
+            const double lr10 = log10(rpres);
+            const double fo = (lr10 + coff) / (noff - doff * (lr10 + coff));
+            const double foff = pow(10.0, pow(1.0 + pow(fo, 2.0), -1.0) * fcent);
+            kf = kinf * (rpres / (1.0 + rpres)) * foff;
+          }
+        } else {
+          const double xoff = 1.0 / (1.0 + pow(log10(rpres), 2.0));
+          const double foff = R[KD_FC + 3] * pow(R[KD_FC] * exp(-R[KD_FC + 1] / tt) + exp(-tt / R[KD_FC + 2]), xoff) *
+                              pow(tt, R[KD_FC + 4]);
+          kf = kinf * (rpres / (1.0 + rpres)) * foff;
+        }
+      }
+    } else if (ty <= 1) {
       kf = R[KD_A];
       if (I[KI_ARR]) kf = kf * exp(R[KD_A + 1] * lnT - R[KD_A + 2] * Rtti);
       if (ty == 1) {
@@ -121,15 +212,21 @@ __device__ inline void chemistry_source_packed(const KinPacked &K, double tt, co
     }
     double q1 = 1.0, q2 = 1.0;
     const int nre = I[KI_NRE], npr = I[KI_NPR];
+    if (REF) {
+      RefSpecies<0, NS>::products(R, I, nre, npr, conc, q1, q2);
+      const double q = tbv * (kf * q1 - kb * q2);
+      RefSpecies<0, NS>::accumulate(R, I, nre, npr, q, mw, acc);
+    } else {
 #pragma unroll
-    for (int e = 0; e < KSIDE; ++e)
-      if (e < nre) q1 = q1 * conc_power(conc[I[KI_RE + e]], R[KD_REX + e]);
+      for (int e = 0; e < KSIDE; ++e)
+        if (e < nre) q1 = q1 * conc_power(conc[I[KI_RE + e]], R[KD_REX + e]);
 #pragma unroll
-    for (int e = 0; e < KSIDE; ++e)
-      if (e < npr) q2 = q2 * conc_power(conc[I[KI_PR + e]], R[KD_PEX + e]);
-    const double q = tbv * (kf * q1 - kb * q2);
+      for (int e = 0; e < KSIDE; ++e)
+        if (e < npr) q2 = q2 * conc_power(conc[I[KI_PR + e]], R[KD_PEX + e]);
+      const double q = tbv * (kf * q1 - kb * q2);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) acc[s] = acc[s] - q * R[KD_NET + s];
+      for (int s = 0; s < NS; ++s) acc[s] = acc[s] - q * R[KD_NET + s];
+    }
   }
 }
 

@@ -1,11 +1,18 @@
-// reactors.hip — batched stiff chemistry: millions of independent 0-D constant-volume
-// reactors integrated in one persistent launch.
+// reactors.hip — batched stiff chemistry: millions of independent 0-D reactors (constant
+// volume, or constant pressure) integrated in one persistent launch.
 //
 // Each reactor is one cell of the reference's rosenbrock_kernel (kernels_gpu.F90:8928-9193):
-// adaptive 4-stage Rosenbrock (Numerical-Recipes ROS4 constants), central-difference
-// Jacobian, Crout LU with implicit pivoting, error scaled by max(1,|w|).  The right-hand side
-// is rosenbrock_rhs (:9249-9339) evaluated with the packed kinetics records (scalar loads)
-// and the NASA tables staged in LDS.  Reactors have uneven stiffness, so a fixed
+// adaptive 4-stage Rosenbrock (Numerical-Recipes ROS4 constants), error scaled by
+// max(1,|w|).  Three forms of it (the `mixed` argument):
+//  0  the plain FP64 integrator, the yardstick of the other two: central-difference
+//     Jacobian over all N_S + 4 variables, FP64 Crout LU with implicit pivoting, and
+//     rosenbrock_rhs (:9249-9339) in the reference's own arithmetic forms (global NASA
+//     tables, reference-form kinetics from the packed records);
+//  1  mixed precision on the species block: one-sided species Jacobian, FP32 LU refined
+//     against the FP64 residual, fast-form kinetics and the NASA tables staged in LDS;
+//  2  form 1 with the LU in FP64 (no refinement): the like-for-like FP64 counterpart of 1,
+//     so that the difference between 1 and 2 is the precision of the factorisation alone.
+// Reactors have uneven stiffness, so a fixed
 // reactor-to-thread assignment would leave lanes idle behind the stiffest reactor of their
 // wave: instead every lane pulls its next reactor from a device-wide counter as soon as it
 // is done, and the grid is sized to the resident capacity of the GPU (persistent).
@@ -92,6 +99,79 @@ struct RosRhsPressure {
   }
 };
 
+// rosenbrock_rhs in the reference's arithmetic: constant volume (the reference's cell)
+template <int NS>
+struct RosRhsPackedRef {
+  const KinPacked &K;
+  const Thermo &th;
+  const double *mw;
+  double tol_iter_nr;
+  __device__ inline void operator()(double *wv, double &tg, double *rhs) const {
+    double yy[NS], conc[NS], acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) wv[s] = wv[s] > 0.0 ? wv[s] : 0.0;
+    double rho = wv[0];
+#pragma unroll
+    for (int s = 1; s < NS; ++s) rho = rho + wv[s];
+    const double ri = 1.0 / rho;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      yy[s] = wv[s] * ri;
+      conc[s] = rho * yy[s] / mw[s];
+      acc[s] = 0.0;
+    }
+    const double uu = wv[NS] * ri, vv = wv[NS + 1] * ri, ww = wv[NS + 2] * ri;
+    const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+    const double ee = wv[NS + 3] * ri - qq;
+    const double tt = temperature_from_e<NS>(ee, tg, th, tol_iter_nr, yy);
+    tg = tt;
+    chemistry_source_packed<NS, true>(K, tt, conc, acc, mw);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rhs[s] = -acc[s];
+#pragma unroll
+    for (int m = NS; m < NS + 4; ++m) rhs[m] = 0.0;
+  }
+};
+
+// The constant-pressure reactor in the same arithmetic: T from h by the e-inversion's
+// Newton iteration on the c_p tables (thh.cv_coeff holds cp_coeff), rho = p / (R T).
+template <int NS>
+struct RosRhsPressureRef {
+  const KinPacked &K;
+  const Thermo &thh;
+  const double *mw, *rgas;
+  double pres, tol_iter_nr;
+  __device__ inline void operator()(double *wv, double &tg, double *rhs) const {
+    double yy[NS], conc[NS], acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) wv[s] = wv[s] > 0.0 ? wv[s] : 0.0;
+    double ysum = wv[0];
+#pragma unroll
+    for (int s = 1; s < NS; ++s) ysum = ysum + wv[s];
+    const double yi = 1.0 / ysum;
+    double rmix = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      yy[s] = wv[s] * yi;
+      rmix = rmix + rgas[s] * yy[s];
+    }
+    const double tt = temperature_from_e<NS>(wv[NS + 3], tg, thh, tol_iter_nr, yy);
+    tg = tt;
+    const double rho = pres / (rmix * tt);
+    const double ri = 1.0 / rho;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      conc[s] = rho * yy[s] / mw[s];
+      acc[s] = 0.0;
+    }
+    chemistry_source_packed<NS, true>(K, tt, conc, acc, mw);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) rhs[s] = -acc[s] * ri;
+#pragma unroll
+    for (int m = NS; m < NS + 4; ++m) rhs[m] = 0.0;
+  }
+};
+
 // Mixed-precision variant of the same method (same ROS4 stages, step-size control and error
 // norm), with three changes:
 //  * The linear algebra runs on the species block.  In a reactor the momentum and energy
@@ -103,7 +183,8 @@ struct RosRhsPressure {
 //    right-hand sides instead of 2 (N_S + 4) + 1).
 //  * W_ss is factored in FP32 and each stage solve is refined once against the FP64
 //    residual b - W x, with J kept in FP64; the right-hand sides, the temperature inversion
-//    and the error estimate stay FP64.
+//    and the error estimate stay FP64.  With LU = double the factorisation is FP64 and the
+//    solves need no refinement (form 2).
 template <int N>
 __device__ inline void solve_refined(const double *jac, const float *lu, const int *indx, double gi, double *b) {
   float x32[N];
@@ -128,7 +209,13 @@ __device__ inline void solve_refined(const double *jac, const float *lu, const i
   for (int m = 0; m < N; ++m) b[m] = x[m] + (double)x32[m];
 }
 
-template <int NS, class Rhs>
+template <int N, class LU>
+__device__ inline void solve_block(const double *jac, const LU *lu, const int *indx, double gi, double *b) {
+  if constexpr (sizeof(LU) == sizeof(double)) lubksb<N, double>(lu, indx, b);
+  else solve_refined<N>(jac, lu, indx, gi, b);
+}
+
+template <int NS, class LU, class Rhs>
 __device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, double dttry, double time_start,
                                    double time_end, int maxsteps, int maxtry, double tol, RosStats &st) {
   constexpr int NV = NS + 4;
@@ -139,7 +226,7 @@ __device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, do
   const double b1 = 19.0 / 9.0, b2 = 1.0 / 2.0, b3 = 25.0 / 108.0, b4 = 125.0 / 108.0;
   const double e1 = 17.0 / 54.0, e2 = 7.0 / 36.0, e3 = 0.0, e4 = 125.0 / 108.0;
   double jac[NS * NS];
-  float lu[NS * NS];
+  LU lu[NS * NS];
   double w_sav[NS], rhs_sav[NV], yscal[NS], rhs[NV], rp[NV];
   double g1[NS], g2[NS], g3[NS], g4[NS];
   int indx[NS];
@@ -165,24 +252,24 @@ __device__ void rosenbrock_cell_mp(const Rhs &rhs_fn, double *wv, double &tg, do
     for (int jtry = 1; jtry <= maxtry; ++jtry) {
       const double gi = 1.0 / (gam * dt);
       for (int n = 0; n < NS; ++n) {
-        for (int m = 0; m < NS; ++m) lu[m + NS * n] = (float)(-jac[m + NS * n]);
-        lu[n + NS * n] = (float)(gi - jac[n + NS * n]);
+        for (int m = 0; m < NS; ++m) lu[m + NS * n] = (LU)(-jac[m + NS * n]);
+        lu[n + NS * n] = (LU)(gi - jac[n + NS * n]);
       }
-      ludcmp<NS, float>(lu, indx);
+      ludcmp<NS, LU>(lu, indx);
       for (int m = 0; m < NS; ++m) g1[m] = rhs_sav[m];
-      solve_refined<NS>(jac, lu, indx, gi, g1);
+      solve_block<NS>(jac, lu, indx, gi, g1);
       for (int m = 0; m < NS; ++m) wv[m] = w_sav[m] + a21 * g1[m];
       time = t0 + dt;
       rhs_fn(wv, tg, rhs);
       for (int m = 0; m < NS; ++m) g2[m] = rhs[m] + c21 * g1[m] / dt;
-      solve_refined<NS>(jac, lu, indx, gi, g2);
+      solve_block<NS>(jac, lu, indx, gi, g2);
       for (int m = 0; m < NS; ++m) wv[m] = w_sav[m] + a31 * g1[m] + a32 * g2[m];
       time = t0 + 3.0 / 5.0 * dt;
       rhs_fn(wv, tg, rhs);
       for (int m = 0; m < NS; ++m) g3[m] = rhs[m] + (c31 * g1[m] + c32 * g2[m]) / dt;
-      solve_refined<NS>(jac, lu, indx, gi, g3);
+      solve_block<NS>(jac, lu, indx, gi, g3);
       for (int m = 0; m < NS; ++m) g4[m] = rhs[m] + (c41 * g1[m] + c42 * g2[m] + c43 * g3[m]) / dt;
-      solve_refined<NS>(jac, lu, indx, gi, g4);
+      solve_block<NS>(jac, lu, indx, gi, g4);
       double errmax = 0.0;
       for (int m = 0; m < NS; ++m) {
         wv[m] = w_sav[m] + b1 * g1[m] + b2 * g2[m] + b3 * g3[m] + b4 * g4[m];
@@ -217,7 +304,7 @@ struct ReactorArgs {
   int *nacc, *nrej;               // per reactor (may be null)
   double *t_done, *dt_next;       // per reactor, resumable mode (both null: from t = 0)
   unsigned *next;                 // work counter (starts at 0)
-  int nbatch, maxsteps, maxtry, mixed;
+  int nbatch, maxsteps, maxtry, mode;   // mode: 0 plain FP64, 1 mixed, 2 FP64 species block
   double tol, tol_iter_nr;
   const double *mw, *rgas;
   const double *pressure;         // per reactor; null: constant volume
@@ -225,7 +312,7 @@ struct ReactorArgs {
   Thermo th;
 };
 
-template <int NS, bool MIXED, class Rhs>
+template <int NS, int MODE, class Rhs>
 __device__ inline void integrate_one(const Rhs &f, const ReactorArgs &A, unsigned r) {
   constexpr int NV = NS + 4;
   const long nb = A.nbatch;
@@ -245,8 +332,12 @@ __device__ inline void integrate_one(const Rhs &f, const ReactorArgs &A, unsigne
       return;
     }
   }
-  if (MIXED) rosenbrock_cell_mp<NS>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
-  else rosenbrock_cell_f<NS>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+  if constexpr (MODE == 1)
+    rosenbrock_cell_mp<NS, float>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+  else if constexpr (MODE == 2)
+    rosenbrock_cell_mp<NS, double>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
+  else
+    rosenbrock_cell_f<NS>(f, wv, tg, dt0, t0, A.time_end[r], A.maxsteps, A.maxtry, A.tol, st);
 #pragma unroll
   for (int m = 0; m < NV; ++m) A.w[(long)m * nb + r] = wv[m];
   A.tg[r] = tg;
@@ -258,22 +349,36 @@ __device__ inline void integrate_one(const Rhs &f, const ReactorArgs &A, unsigne
   }
 }
 
-template <int NS, int LO, bool MIXED, bool CP>
+template <int NS, int LO, int MODE, bool CP>
 __global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
   constexpr int WHICH = CP ? (TH_CP | TH_CPH) : (TH_CV | TH_CVH);
-  __shared__ double tlds[thermo_lds_doubles<NS>(WHICH)];
-  const ThermoLds TL = thermo_stage<NS, LO>(A.th, tlds, WHICH);
-  __syncthreads();
+  // the plain FP64 form reads the NASA tables from global memory in the reference's forms
+  __shared__ double tlds[MODE == 0 ? 1 : thermo_lds_doubles<NS>(WHICH)];
+  ThermoLds TL{};
+  if constexpr (MODE != 0) {
+    TL = thermo_stage<NS, LO>(A.th, tlds, WHICH);
+    __syncthreads();
+  }
+  Thermo thh = A.th;
+  thh.cv_coeff = A.th.cp_coeff;
   const long nb = A.nbatch;
   for (;;) {
     const unsigned r = atomicAdd(A.next, 1u);
     if ((long)r >= nb) break;
-    if (CP) {
+    if constexpr (MODE == 0) {
+      if (CP) {
+        const RosRhsPressureRef<NS> f{A.K, thh, A.mw, A.rgas, A.pressure[r], A.tol_iter_nr};
+        integrate_one<NS, MODE>(f, A, r);
+      } else {
+        const RosRhsPackedRef<NS> f{A.K, A.th, A.mw, A.tol_iter_nr};
+        integrate_one<NS, MODE>(f, A, r);
+      }
+    } else if (CP) {
       const RosRhsPressure<NS, LO> f{A.K, TL, A.mw, A.rgas, A.pressure[r], A.tol_iter_nr};
-      integrate_one<NS, MIXED>(f, A, r);
+      integrate_one<NS, MODE>(f, A, r);
     } else {
       const RosRhsPacked<NS, LO> f{A.K, TL, A.mw, A.tol_iter_nr};
-      integrate_one<NS, MIXED>(f, A, r);
+      integrate_one<NS, MODE>(f, A, r);
     }
   }
 }
@@ -299,29 +404,37 @@ inline int reactor_grid(int wg_per_cu, long nbatch) {
   return (int)(g < need ? g : need);
 }
 
-template <int NS, int LO, bool MIXED, bool CP>
+template <int NS, int LO, int MODE, bool CP>
 int reactor_launch_t(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
   const int g = reactor_grid(wg_per_cu, A.nbatch);
   if (g <= 0) return STREAMS_E_DIMS;
-  reactor_kernel<NS, LO, MIXED, CP><<<g, 256, 0, st>>>(A);
+  reactor_kernel<NS, LO, MODE, CP><<<g, 256, 0, st>>>(A);
   return hip_status();
+}
+
+template <int NS, int LO, int MODE>
+int reactor_launch_m(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
+  return A.pressure != nullptr ? reactor_launch_t<NS, LO, MODE, true>(A, wg_per_cu, st)
+                               : reactor_launch_t<NS, LO, MODE, false>(A, wg_per_cu, st);
 }
 
 template <int NS, int LO>
 int reactor_launch(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
-  const bool cp = A.pressure != nullptr;
-  if (A.mixed) return cp ? reactor_launch_t<NS, LO, true, true>(A, wg_per_cu, st)
-                         : reactor_launch_t<NS, LO, true, false>(A, wg_per_cu, st);
-  return cp ? reactor_launch_t<NS, LO, false, true>(A, wg_per_cu, st)
-            : reactor_launch_t<NS, LO, false, false>(A, wg_per_cu, st);
+  switch (A.mode) {
+    case 0: return reactor_launch_m<NS, LO, 0>(A, wg_per_cu, st);
+    case 1: return reactor_launch_m<NS, LO, 1>(A, wg_per_cu, st);
+    case 2: return reactor_launch_m<NS, LO, 2>(A, wg_per_cu, st);
+    default: return STREAMS_E_OPTION;
+  }
 }
 
 }  // namespace streams
 
 using namespace streams;
 
-// Batched integration of nbatch independent reactors (rosenbrock_kernel per reactor, or its
-// mixed-precision variant when mixed != 0) from t = 0 to time_end[b]: constant volume when
+// Batched integration of nbatch independent reactors (rosenbrock_kernel per reactor with
+// mixed = 0, the mixed-precision species-block variant with mixed = 1, its FP64 counterpart
+// with mixed = 2) from t = 0 to time_end[b]: constant volume when
 // pressure is null, else constant pressure pressure[b] with the state (Y, 0, 0, 0, h).  With
 // t_done / dt_next (both or neither) the call is resumable: each reactor starts at
 // t_done[b] with the step dt_next[b] (dttry[b] when not positive), takes at most maxsteps
@@ -337,6 +450,7 @@ extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tgu
                                      int maxtry, double tol, int *nacc, int *nrej, double *t_done, double *dt_next,
                                      unsigned *work, int wg_per_cu, int mixed, void *stream) {
   if (nbatch < 0 || nreactions < 0 || maxsteps < 1 || maxtry < 1) return STREAMS_E_DIMS;
+  if (mixed < 0 || mixed > 2) return STREAMS_E_OPTION;
   if (n_s > KTB) return STREAMS_E_NSPECIES;
   if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;
   if (nbatch == 0) return 0;
@@ -344,7 +458,7 @@ extern "C" int streams_reactor_batch(int n_s, int nbatch, double *w, double *tgu
   hipStream_t st = as_stream(stream);
   if (pressure != nullptr && (rgas == nullptr || cp_coeff == nullptr)) return STREAMS_E_DIMS;
   if ((t_done == nullptr) != (dt_next == nullptr)) return STREAMS_E_DIMS;
-  ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, t_done, dt_next, work, nbatch, maxsteps, maxtry, mixed != 0, tol,
+  ReactorArgs A{w, tguess, dttry, time_end, nacc, nrej, t_done, dt_next, work, nbatch, maxsteps, maxtry, mixed, tol,
                 tol_iter_nr, mw,
                 rgas, pressure, KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
                 Thermo{cv_coeff, cp_coeff, trange, nsetcv, indx_cp_l, indx_cp_r}};

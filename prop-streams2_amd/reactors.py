@@ -1,9 +1,15 @@
-"""Batched 0-D constant-volume reactors on the GPU (device-resident state).
+"""Batched 0-D reactors on the GPU (device-resident state), constant volume or constant
+pressure.
 
 Each reactor is one cell of the reference's rosenbrock_kernel (kernels_gpu.F90:8928-9193)
 integrated from t = 0 to its own horizon; streams_reactor_batch runs any number of them in
 one persistent launch with a device-wide work counter, so reactors of very different
-stiffness keep every lane busy.
+stiffness keep every lane busy.  Three forms of the integrator (``mixed``):
+
+* 0 (False): the plain FP64 integrator in the reference's arithmetic, the yardstick;
+* 1 (True): mixed precision on the species block (one-sided species Jacobian, FP32 LU
+  refined against the FP64 residual);
+* 2: form 1 with an FP64 LU, its like-for-like FP64 counterpart.
 """
 from __future__ import annotations
 
@@ -38,11 +44,12 @@ class ReactorBatchGPU:
         self.work = backend.zeros((2,), "i4")
 
     def integrate(self, w, tg, dttry, time_end, nacc=None, nrej=None, wg_per_cu: int = 0, stream=None,
-                  mixed: bool = False, pressure=None, work=None, t_done=None, dt_next=None,
+                  mixed=False, pressure=None, work=None, t_done=None, dt_next=None,
                   maxsteps: Optional[int] = None):
         """w (nv, nb) float64, tg (nb) float64, dttry/time_end (nb) float64, all on the device;
         nacc/nrej optional int32 (nb).  Enqueued on `stream` (default: the backend's).
-        mixed=True: the species-block integrator (FP32 factor, FP64 refinement).
+        mixed: 0/False the plain FP64 integrator, 1/True the species-block integrator
+        (FP32 factor, FP64 refinement), 2 the species-block integrator in FP64.
         pressure (nb) on the device: constant-pressure reactors with w = (Y, 0, 0, 0, h).
         work: a private counter for launches that may overlap on other streams.
         t_done/dt_next (nb) float64 on the device: resumable call, at most `maxsteps` steps

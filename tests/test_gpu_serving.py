@@ -17,7 +17,7 @@ def setup(pkg):
     hip = pkg.solver.HipBackend()
     mechs = {"h2": pkg.cases.h2_air_mechanism().tables(),
              "syngas": pkg.mechanism.Mechanism(
-                 str(oracle_backend.ROOT / "tests/golden/mechanisms/keromnes2013_14sp.yaml")).tables()}
+                 str(pkg.cases.mechanism_path("keromnes2013_14sp.yaml"))).tables()}
     fuels = {"h2": {"H2": 1.0}, "syngas": {"H2": 0.5, "CO": 0.5}}
     reqs = pkg.serving.poisson_requests(mechs, rate=400.0, duration=0.05, seed=9, max_reactors=48,
                                         horizon=(2e-5, 2e-4), t0=(1100.0, 1600.0), fuels=fuels)
