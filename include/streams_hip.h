@@ -60,6 +60,52 @@ int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int v
                       const double *detady, const double *dzitdz, double eps_sensor,
                       int sensor_type, void *stream);
 
+/* ---- LES auxiliaries (compute_aux_les, multideal_gpu.F90:445-498) ------------------ */
+
+/* eval_velaux_cuf, kernels_gpu.F90:9879-9917: over the padded box, species clipped at 0
+ * in w, Y, rho, u, v, w into w_aux and the eddy-viscosity slot J_LES1 = 14 + 2 N_S cleared.
+ * nv_aux >= 2 n_s + 14. */
+int streams_eval_velaux(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux,
+                        double *w, double *w_aux, void *stream);
+
+/* les_wale_mut_cuf, kernels_gpu.F90:9921-10047: on the interior, the velocity gradient by
+ * central differences of order ep_order (lmax = ep_order / 2 <= min(4, ng)), the Ducros
+ * sensor and div/3 as eval_aux2 computes them, and the WALE eddy viscosity
+ * rho C_w^2 Delta^2 |S^d|^3 / (|S|^5 + |S^d|^(5/2)) into J_LES1. */
+int streams_les_wale_mut(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int ep_order,
+                         double *w_aux, const double *coeff_deriv1, const double *dcsidx,
+                         const double *detady, const double *dzitdz, double les_c_wale,
+                         double eps_sensor, int sensor_type, void *stream);
+
+/* eval_aux_les_cuf, kernels_gpu.F90:10049-10161: over the padded box, from the rho, u, v,
+ * w and Y in w_aux: T, h, p, c and the mixture transport of eval_aux with the eddy
+ * viscosity added (mu + mu_sgs, k + mu_sgs c_p / les_pr, rho D_s + mu_sgs / les_sc); the
+ * mixture fraction J_Z is left as it is. */
+int streams_eval_aux_les(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux,
+                         double *w, double *w_aux, double p0, double t_min_tab, double dt_tab,
+                         double r_univ, const double *rgas, const double *cv_coeff,
+                         const double *cp_coeff, int nsetcv, const double *trange,
+                         int indx_cp_l, int indx_cp_r, double tol_iter_nr, const double *mw,
+                         const double *mwinv, const double *visc_species,
+                         const double *lambda_species, const double *diffbin_species,
+                         int num_t_tab, double les_pr, double les_sc, void *stream);
+
+/* eval_chem_aux_cuf, kernels_gpu.F90:9676-9877 (compute_chem_aux, multideal_gpu.F90:500-534,
+ * at output times): on the interior, the production rates sum_r (p - r) q_r W_s at the
+ * rho, T, Y of w_aux into J_WDOT_START + s (scaled by the PaSR factor when enable_pasr)
+ * and the heat release rate -sum_s h298_s wdot_s / W_s into J_HRR = 15 + 3 N_S (the slot
+ * the reference also gives the PaSR factor, which the heat release rate overwrites).
+ * nv_aux >= 3 n_s + 15. */
+int streams_eval_chem_aux(int n_s, int nx, int ny, int nz, int nv_aux, int ng, int nreactions,
+                          double *w_aux, const double *mw, const double *arr_a,
+                          const double *arr_b, const double *arr_ea, const double *tb_eff,
+                          const double *falloff_coeffs, const int *reac_ty, const int *isrev,
+                          const double *r_coeffs, const double *p_coeffs, const double *kc_tab,
+                          int num_t_tab, double t_min_tab, double dt_tab, double r_univ,
+                          const double *dcsidx, const double *detady, const double *dzitdz,
+                          const double *h298, int enable_pasr, double les_c_yoshi,
+                          double les_c_mix, double les_c_eps, void *stream);
+
 /* ---- ghost cells and halo exchange (a3, a11) --------------------------------------- */
 
 /* bcextr_var_cuf, kernels_gpu.F90:5147-5207: zero-order extrapolation of one variable
@@ -192,7 +238,10 @@ int streams_compute_chemistry(int n_s, int nx, int ny, int nz, int nv, int nv_au
 /* rosenbrock_kernel, kernels_gpu.F90:8928-9501 (launched by rosenbrock,
  * multideal_gpu.F90:694-767): per cell adaptive 4-stage Rosenbrock from time_start to
  * time_end with a central-difference Jacobian and partially pivoted LU, all FP64.
- * enable_pasr must be 0 (the PaSR closure belongs to the LES path). */
+ * enable_pasr = 1 scales the species sources of every right-hand side by the PaSR factor
+ * tau_c / (tau_c + tau_k), tau_k from w_aux's rho, mu and eddy viscosity (J_LES1, so
+ * nv_aux >= 2 n_s + 14) and the metrics dcsidx, detady, dzitdz (kernels_gpu.F90:8970-8980,
+ * 9359-9379). */
 int streams_rosenbrock(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
                        double *w, double *w_aux, const double *fl, const double *mw,
                        int nreactions, const double *arr_a, const double *arr_b,
@@ -214,7 +263,9 @@ int streams_rosenbrock(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int 
  * finite-difference Jacobian-vector products for every attempted step, error norm
  * ||(yt - ys) / (1e-4 ys + 1e-8)||_2, only for cells with 5e-4 < Z < 1 - 5e-4.
  * simpler_splitting = 1 subtracts the transport forcing fl_sav(1:nx,1:ny,1:nz,1:nv) from
- * every right-hand side.  w_aux is read only (T guess, Z).  enable_pasr must be 0. */
+ * every right-hand side.  w_aux is read only (T guess, Z).  enable_pasr = 1 applies the
+ * PaSR factor to every right-hand side as in streams_rosenbrock (:10475-10486,
+ * 10954-10974). */
 int streams_rosenbrock_krylov(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
                               double *w, double *w_aux, const double *fl, const double *mw,
                               int nreactions, const double *arr_a, const double *arr_b,
@@ -264,6 +315,7 @@ int streams_compute_dt(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_a
                        const double *w, const double *w_aux, double *dtmax9,
                        int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
                        const double *trange, const int *fluid_mask, void *stream);
+
 
 
 /* ---- host <-> device field copies and checkpoints (base_gpu.F90:1266-1325) ---------- */

@@ -24,6 +24,10 @@ STATUS = {-1: "N_S not compiled", -2: "inconsistent sizes", -3: "unsupported opt
 _ARGS = {
     "eval_aux": "iiiiiiiiiiiii" "pp" "dddd" "ppp" "ip" "iid" "s" "ppppp" "i" "i" "pppp",
     "eval_aux2": "iiiiiii" "p" "pppp" "d" "i" "s",
+    "eval_velaux": "iiiiiii" "pp" "s",
+    "les_wale_mut": "iiiiiii" "p" "pppp" "dd" "i" "s",
+    "eval_aux_les": "iiiiiii" "pp" "dddd" "ppp" "i" "p" "ii" "d" "ppppp" "i" "dd" "s",
+    "eval_chem_aux": "iiiiiii" "p" "p" "ppp" "pp" "pp" "ppp" "i" "ddd" "ppp" "p" "i" "ddd" "s",
     "bcextr_var": "iiii" "p" "s",
     "bcswap": "iiiii" "p" "p" "s",
     "bcextr": "iiiiii" "p" "s",

@@ -24,7 +24,11 @@ struct AuxTables {
 // and the mixture transport (:6252-6300) run as two kernels over the same points: apart,
 // each holds far fewer registers than the pair would, for the price of re-reading Y, T, p
 // and rho (12 of the ~45 doubles a point moves).
-template <int NS, int LO>
+// LES = true is eval_aux_les_cuf (kernels_gpu.F90:10049-10161): rho, u, v, w and Y come
+// from w_aux (left there by eval_velaux), nothing of them is written back, the mixture
+// fraction is not touched, and c_p is left in the J_K_COND slot for the transport kernel,
+// which overwrites it with the conductivity.
+template <int NS, int LO, bool LES = false>
 __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int jstart, int kstart,
                                                        int ni, int nj, long npts, double *__restrict__ w,
                                                        double *__restrict__ w_aux, Thermo th,
@@ -41,26 +45,38 @@ __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int js
   const long pt = b.p(i, j, k), ps = b.pstride();
 
   double yy[NS];
-  double rho = 0.0;
+  double rho = 0.0, uu, vv, ww;
+  if (LES) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    double v = w[pt + ps * s];
-    if (!(v > 0.0)) {               // max(rho*Y, 0): written back only when clipped
-      v = 0.0;
-      w[pt + ps * s] = 0.0;
+    for (int s = 0; s < NS; ++s) yy[s] = w_aux[pt + ps * s];
+    rho = w_aux[pt + ps * (X::J_R - 1)];
+    uu = w_aux[pt + ps * (X::J_U - 1)];
+    vv = w_aux[pt + ps * (X::J_V - 1)];
+    ww = w_aux[pt + ps * (X::J_W - 1)];
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double v = w[pt + ps * s];
+      if (!(v > 0.0)) {               // max(rho*Y, 0): written back only when clipped
+        v = 0.0;
+        w[pt + ps * s] = 0.0;
+      }
+      yy[s] = v;
+      rho = (s == 0) ? v : rho + v;
     }
-    yy[s] = v;
-    rho = (s == 0) ? v : rho + v;
   }
   const double ri = 1.0 / rho;
+  if (!LES) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    yy[s] = yy[s] * ri;
-    w_aux[pt + ps * s] = yy[s];
+    for (int s = 0; s < NS; ++s) {
+      yy[s] = yy[s] * ri;
+      w_aux[pt + ps * s] = yy[s];
+    }
+    uu = w[pt + ps * (X::I_U - 1)] * ri;
+    vv = w[pt + ps * (X::I_V - 1)] * ri;
+    ww = w[pt + ps * (X::I_W - 1)] * ri;
   }
-  const double rhou = w[pt + ps * (X::I_U - 1)], rhov = w[pt + ps * (X::I_V - 1)];
-  const double rhow = w[pt + ps * (X::I_W - 1)], rhoe = w[pt + ps * (X::I_E - 1)];
-  const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+  const double rhoe = w[pt + ps * (X::I_E - 1)];
   const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
   const double ee = rhoe * ri - qq;
   int nrange[NS];
@@ -74,6 +90,14 @@ __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int js
   const double gamloc = cploc / (cploc - rmix);
   const double pp = rho * tt * rmix;
   const double c = sqrt(gamloc * pp * ri);
+  if (LES) {
+    w_aux[pt + ps * (X::J_H - 1)] = (rhoe + pp) * ri;
+    w_aux[pt + ps * (X::J_T - 1)] = tt;
+    w_aux[pt + ps * (X::J_P - 1)] = pp;
+    w_aux[pt + ps * (X::J_C - 1)] = c;
+    w_aux[pt + ps * (X::J_K_COND - 1)] = cploc;
+    return;
+  }
   w_aux[pt + ps * (X::J_R - 1)] = rho;
   w_aux[pt + ps * (X::J_U - 1)] = uu;
   w_aux[pt + ps * (X::J_V - 1)] = vv;
@@ -98,10 +122,13 @@ __global__ void __launch_bounds__(256) eval_aux_kernel(Box b, int istart, int js
   w_aux[pt + ps * (X::J_Z - 1)] = zz;
 }
 
-template <int NS>
+// LES = true adds the eddy viscosity mu_sgs (J_LES1) as eval_aux_les_cuf does: mu + mu_sgs,
+// k + mu_sgs c_p / Pr_sgs (c_p left in J_K_COND by the thermo kernel), rho D_s + mu_sgs / Sc_sgs.
+template <int NS, bool LES = false>
 __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, int jstart, int kstart, int ni,
                                                              int nj, long npts, double *__restrict__ w_aux,
-                                                             AuxTables tb) {
+                                                             AuxTables tb, double les_pr = 0.0,
+                                                             double les_sc = 0.0) {
   using X = Idx<NS>;
   __shared__ double c_inv[NS * NS];  // 1/sqrt(1 + W_l/W_m)
   __shared__ double c_pow[NS * NS];  // (W_m/W_l)^(1/4)
@@ -172,8 +199,16 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
     __builtin_amdgcn_sched_barrier(0);
   }
   mu = sqrt(8.0) * mu;
-  w_aux[pt + ps * (X::J_MU - 1)] = mu;
-  w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond;
+  double mu_sgs = 0.0;
+  if (LES) {
+    mu_sgs = w_aux[pt + ps * (X::J_LES1 - 1)];
+    const double cploc = w_aux[pt + ps * (X::J_K_COND - 1)];
+    w_aux[pt + ps * (X::J_MU - 1)] = mu + mu_sgs;
+    w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond + mu_sgs * cploc / les_pr;
+  } else {
+    w_aux[pt + ps * (X::J_MU - 1)] = mu;
+    w_aux[pt + ps * (X::J_K_COND - 1)] = k_cond;
+  }
   // Bird species diffusion into the mixture (mole fractions rebuilt here rather than kept
   // live through the Wilke sums; the fence keeps the table reads below from being hoisted
   // into the Wilke loop)
@@ -196,7 +231,8 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
         dden = dden + xs[m] * rcp_nr(dij);
       }
     }
-    w_aux[pt + ps * (X::J_D_START + l)] = (dden > 1.0e-15) ? (1.0 - yy[l]) / dden * fac : 0.0;
+    const double rd = (dden > 1.0e-15) ? (1.0 - yy[l]) / dden * fac : 0.0;
+    w_aux[pt + ps * (X::J_D_START + l)] = LES ? rd + mu_sgs / les_sc : rd;
   }
 }
 
@@ -309,5 +345,29 @@ extern "C" int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv
                                       : eval_aux2_kernel<NS, 4><<<gm, 256, 0, as_stream(stream)>>>(
                                             b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
                                             sensor_type, SR)));
+  return hip_status();
+}
+
+extern "C" int streams_eval_aux_les(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux, double *w,
+                                    double *w_aux, double p0, double t_min_tab, double dt_tab, double r_univ,
+                                    const double *rgas, const double *cv_coeff, const double *cp_coeff, int nsetcv,
+                                    const double *trange, int indx_cp_l, int indx_cp_r, double tol_iter_nr,
+                                    const double *mw, const double *mwinv, const double *visc_species,
+                                    const double *lambda_species, const double *diffbin_species, int num_t_tab,
+                                    double les_pr, double les_sc, void *stream) {
+  if (nv != n_s + 4 || nv_aux < 2 * n_s + 14) return STREAMS_E_DIMS;
+  if (nx < 1 || ny < 1 || nz < 1 || ng < 0 || num_t_tab < 1) return STREAMS_E_DIMS;
+  if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;
+  Box b{nx, ny, nz, ng};
+  Thermo th{cv_coeff, cp_coeff, trange, nsetcv, indx_cp_l, indx_cp_r};
+  AuxTables tb{rgas, mw, mwinv, visc_species, lambda_species, diffbin_species, num_t_tab, t_min_tab,
+               dt_tab, p0, r_univ, 1, nullptr, nullptr, nullptr, nullptr};
+  const int ni = nx + 2 * ng, nj = ny + 2 * ng;
+  const long npts = b.pstride();
+  STREAMS_DISPATCH_NS(n_s, STREAMS_DISPATCH_NASA(indx_cp_l, indx_cp_r,
+                               (eval_aux_kernel<NS, LO, true><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
+                                   b, 1 - ng, 1 - ng, 1 - ng, ni, nj, npts, w, w_aux, th, tb, tol_iter_nr),
+                                eval_transport_kernel<NS, true><<<blocks_for(npts, 256), 256, 0, as_stream(stream)>>>(
+                                   b, 1 - ng, 1 - ng, 1 - ng, ni, nj, npts, w_aux, tb, les_pr, les_sc))));
   return hip_status();
 }

@@ -69,7 +69,7 @@ template <int NS>
 __global__ void __launch_bounds__(128) rosenbrock_kernel(Box b, double *__restrict__ w, double *__restrict__ w_aux,
                                                          Kin K, Thermo th, double tol_iter_nr, double dttry,
                                                          double time_start, double time_end, int maxsteps,
-                                                         int maxtry, double tol) {
+                                                         int maxtry, double tol, Pasr pasr) {
   using X = Idx<NS>;
   constexpr int NV = NS + 4;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,8 +83,9 @@ __global__ void __launch_bounds__(128) rosenbrock_kernel(Box b, double *__restri
 #pragma unroll
   for (int m = 0; m < NV; ++m) wv[m] = w[pt + ps * m];
   double tg = w_aux[pt + ps * (X::J_T - 1)];
+  const double tau_k = pasr.on ? pasr.tau_k<NS>(w_aux, pt, ps, i, j, k) : 0.0;
   RosStats st;
-  rosenbrock_cell<NS>(K, th, tol_iter_nr, wv, tg, dttry, time_start, time_end, maxsteps, maxtry, tol, st);
+  rosenbrock_cell<NS>(K, th, tol_iter_nr, wv, tg, dttry, time_start, time_end, maxsteps, maxtry, tol, st, tau_k);
 #pragma unroll
   for (int m = 0; m < NV; ++m) w[pt + ps * m] = wv[m];
   w_aux[pt + ps * (X::J_T - 1)] = tg;
@@ -126,10 +127,10 @@ extern "C" int streams_rosenbrock(int n_s, int nv, int nv_aux, int nx, int ny, i
                                   int simpler_splitting, int enable_pasr, double les_c_yoshi, double les_c_mix,
                                   double les_c_eps, const double *dcsidx, const double *detady,
                                   const double *dzitdz, void *stream) {
-  (void)fl; (void)simpler_splitting; (void)les_c_yoshi; (void)les_c_mix; (void)les_c_eps;
-  (void)dcsidx; (void)detady; (void)dzitdz;
-  if (enable_pasr != 0) return STREAMS_E_OPTION;
+  (void)fl; (void)simpler_splitting;
   if (nv != n_s + 4 || nv_aux < 2 * n_s + 13 || maxsteps < 1 || maxtry < 1) return STREAMS_E_DIMS;
+  if (enable_pasr > 0 && (nv_aux < 2 * n_s + 14 || !dcsidx || !detady || !dzitdz)) return STREAMS_E_DIMS;
+  const Pasr pasr{enable_pasr > 0, les_c_yoshi, les_c_mix, les_c_eps, dcsidx, detady, dzitdz};
   Box b{nx, ny, nz, ng};
   long n = b.fstride();
   if (n == 0) return 0;
@@ -138,6 +139,6 @@ extern "C" int streams_rosenbrock(int n_s, int nv, int nv_aux, int nx, int ny, i
   Thermo th{cv_coeff, nullptr, trange, nsetcv, indx_cp_l, indx_cp_r};
   STREAMS_DISPATCH_NS(n_s, (rosenbrock_kernel<NS><<<blocks_for(n, 128), 128, 0, as_stream(stream)>>>(
                                 b, w, w_aux, K, th, tol_iter_nr, dttry, time_start, time_end, maxsteps, maxtry,
-                                tol)));
+                                tol, pasr)));
   return hip_status();
 }

@@ -7,6 +7,8 @@
 // The integrator restates rosenbrock_kernel/_step/_jacobian (:8928-9193) with the
 // Numerical-Recipes ROS4 constants and ludcmp/lubksb (:9395-9501).
 #pragma once
+#include <cfloat>
+
 #include "common.h"
 
 namespace streams {
@@ -94,8 +96,61 @@ __device__ inline double progress_rate(const Kin &K, int lr, double tt, double R
 }
 
 // rosenbrock_rhs on a cell state wv[NV] (species clipped in place, T guess updated)
+// The PaSR closure of the LES path (rosenbrock_rhs, kernels_gpu.F90:9359-9379, and
+// rosenbrock_krylov_rhs, :10954-10974): tau_c is the geometric mean of the slowest and the
+// fastest species time Y_s / |wdot_s / rho| over the species with |wdot_s / rho| > 1e-8,
+// and the species sources are scaled by tau_c / (tau_c + tau_k).  With tau_k not above the
+// smallest normal double the factor is 1 and the call is skipped.
 template <int NS>
-__device__ void ros_rhs(const Kin &K, const Thermo &th, double tol_iter_nr, double *wv, double &tg, double *rhs) {
+__device__ inline void pasr_scale(double tau_k, double ri, const double *yy, double *rhs) {
+  double tau_cf = 0.0, tau_cs = DBL_MAX;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double a = fabs(rhs[s] * ri);
+    if (a > 1.0e-8) {
+      const double tau = yy[s] / a;
+      tau_cf = tau > tau_cf ? tau : tau_cf;
+      if (tau > 0.0 && tau < tau_cs) tau_cs = tau;
+    }
+  }
+  const double tau_c = sqrt(tau_cs * tau_cf);
+  const double gam = tau_c / (tau_c + tau_k);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rhs[s] = gam * rhs[s];
+}
+
+// The cell's mixing time of the PaSR closure (kernels_gpu.F90:8970-8980):
+// Delta = (dxi/dx deta/dy dzeta/dz)^(-1/3), eps_sgs = c_eps c_yoshi^(3/2) nu_sgs^3 / Delta^4,
+// tau_k = c_mix sqrt(nu_eff / eps_sgs) with nu_sgs = mu_sgs / rho and nu_eff = mu / rho.
+__device__ inline double pasr_tau_k(double rho, double mu_sgs, double mu, double dx, double dy, double dz,
+                                    double c_yoshi, double c_mix, double c_eps) {
+  const double ri = 1.0 / rho;
+  const double delta = pow(dx * dy * dz, -1.0 / 3.0);
+  const double nu_sgs = mu_sgs * ri;
+  const double d2 = delta * delta;
+  const double eps_sgs = c_eps * pow(c_yoshi, 1.5) * (nu_sgs * nu_sgs * nu_sgs) / (d2 * d2);
+  const double r = (mu * ri) / (eps_sgs + DBL_MIN);
+  return c_mix * sqrt(r > 0.0 ? r : 0.0);
+}
+
+// The closure's constants and metrics as the integrator kernels take them (enable_pasr,
+// les_c_yoshi, les_c_mix, les_c_eps, dcsidx, detady, dzitdz of rosenbrock_kernel).
+struct Pasr {
+  bool on;
+  double c_yoshi, c_mix, c_eps;
+  const double *dcsidx, *detady, *dzitdz;
+  // tau_k of interior cell (i, j, k) from rho, mu_sgs and mu in w_aux (point pt, stride ps)
+  template <int NS>
+  __device__ inline double tau_k(const double *w_aux, long pt, long ps, int i, int j, int k) const {
+    using X = Idx<NS>;
+    return pasr_tau_k(w_aux[pt + ps * (X::J_R - 1)], w_aux[pt + ps * (X::J_LES1 - 1)], w_aux[pt + ps * (X::J_MU - 1)],
+                      dcsidx[i - 1], detady[j - 1], dzitdz[k - 1], c_yoshi, c_mix, c_eps);
+  }
+};
+
+template <int NS>
+__device__ void ros_rhs(const Kin &K, const Thermo &th, double tol_iter_nr, double *wv, double &tg, double *rhs,
+                        double tau_k = 0.0) {
   constexpr int NV = NS + 4;
   using X = Idx<NS>;
   double yy[NS], conc[NS];
@@ -131,6 +186,7 @@ __device__ void ros_rhs(const Kin &K, const Thermo &th, double tol_iter_nr, doub
       if (rr != 0.0 || pp != 0.0) rhs[s] = rhs[s] + ((pp - rr) * q) * K.mw[s];
     }
   }
+  if (tau_k > DBL_MIN) pasr_scale<NS>(tau_k, ri, yy, rhs);
 }
 
 // ludcmp: Crout LU with implicit partial pivoting, a is n x n column-major.  A row whose
@@ -301,16 +357,17 @@ struct RosRhsRef {
   const Kin &K;
   const Thermo &th;
   double tol_iter_nr;
+  double tau_k;   // PaSR mixing time; 0 without the closure
   __device__ inline void operator()(double *wv, double &tg, double *out) const {
-    ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, out);
+    ros_rhs<NS>(K, th, tol_iter_nr, wv, tg, out, tau_k);
   }
 };
 
 template <int NS>
 __device__ void rosenbrock_cell(const Kin &K, const Thermo &th, double tol_iter_nr, double *wv, double &tg,
                                 double dttry, double time_start, double time_end, int maxsteps, int maxtry,
-                                double tol, RosStats &st) {
-  const RosRhsRef<NS> f{K, th, tol_iter_nr};
+                                double tol, RosStats &st, double tau_k = 0.0) {
+  const RosRhsRef<NS> f{K, th, tol_iter_nr, tau_k};
   rosenbrock_cell_f<NS>(f, wv, tg, dttry, time_start, time_end, maxsteps, maxtry, tol, st);
 }
 

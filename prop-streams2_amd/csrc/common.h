@@ -52,6 +52,9 @@ template <int NS> struct Idx {
   static constexpr int J_U = NS + 1, J_V = NS + 2, J_W = NS + 3, J_H = NS + 4, J_T = NS + 5;
   static constexpr int J_P = NS + 6, J_C = NS + 7, J_MU = NS + 8, J_DUC = NS + 9, J_DIV = NS + 10;
   static constexpr int J_R = NS + 11, J_Z = NS + 12, J_K_COND = NS + 13, J_D_START = NS + 13;
+  // LES eddy viscosity, production rates and heat release (index_define.h:22-26; the PaSR
+  // factor shares the heat-release slot)
+  static constexpr int J_LES1 = 2 * NS + 14, J_WDOT_START = 2 * NS + 14, J_HRR = 3 * NS + 15;
   static constexpr int NV = NS + 4;
 };
 

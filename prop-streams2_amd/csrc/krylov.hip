@@ -18,13 +18,13 @@ constexpr int KSTAGE = 4;   // stages
 // rosenbrock_krylov_rhs: the state is not modified and the temperature guess is not updated
 template <int NS>
 __device__ inline void kry_rhs(const Kin &K, const Thermo &th, double tol_iter_nr, const double *w, double t_iter,
-                               double *rhs) {
+                               double *rhs, double tau_k) {
   constexpr int NV = NS + 4;
   double tmp[NV];
 #pragma unroll
   for (int m = 0; m < NV; ++m) tmp[m] = w[m];
   double tg = t_iter;
-  ros_rhs<NS>(K, th, tol_iter_nr, tmp, tg, rhs);
+  ros_rhs<NS>(K, th, tol_iter_nr, tmp, tg, rhs, tau_k);
 }
 
 template <int N>
@@ -69,7 +69,8 @@ __device__ inline void kry_inv(double *a, int d) {
 // |dt gam H(kk+1,kk)| |lambda(kk)| < 1e-6; Vb and inv stay zero when no size passes.
 template <int NS>
 __device__ void kry_arnoldi(const Kin &K, const Thermo &th, double tol_iter_nr, const double *w_sav, double dt,
-                            double t_iter, const double *fls, double *Vb, double *inv, double *rhs_sav) {
+                            double t_iter, const double *fls, double *Vb, double *inv, double *rhs_sav,
+                            double tau_k) {
   constexpr int NV = NS + 4;
   const double gam = 0.572816062482135, TOL = 1.0e-6, eps = 1.0e-08;
   double HH[(KM + 1) * KM], V[NV * (KM + 1)], HHinv[KM * KM], rpr[KM], jac[NV], wl[NV];
@@ -77,14 +78,14 @@ __device__ void kry_arnoldi(const Kin &K, const Thermo &th, double tol_iter_nr, 
   for (int q = 0; q < NV * (KM + 1); ++q) V[q] = 0.0;
   for (int q = 0; q < KM * KM; ++q) { HHinv[q] = 0.0; inv[q] = 0.0; }
   for (int q = 0; q < NV * KM; ++q) Vb[q] = 0.0;
-  kry_rhs<NS>(K, th, tol_iter_nr, w_sav, t_iter, rhs_sav);
+  kry_rhs<NS>(K, th, tol_iter_nr, w_sav, t_iter, rhs_sav, tau_k);
   if (fls)
     for (int m = 0; m < NV; ++m) rhs_sav[m] = rhs_sav[m] - fls[m];
   const double nrm = norm2v<NV>(rhs_sav);
   for (int m = 0; m < NV; ++m) V[m] = rhs_sav[m] / nrm;
   for (int kk = 1; kk <= KM; ++kk) {
     for (int m = 0; m < NV; ++m) wl[m] = w_sav[m] + eps * V[m + NV * (kk - 1)];
-    kry_rhs<NS>(K, th, tol_iter_nr, wl, t_iter, jac);
+    kry_rhs<NS>(K, th, tol_iter_nr, wl, t_iter, jac, tau_k);
     if (fls)
       for (int m = 0; m < NV; ++m) jac[m] = jac[m] - fls[m];
     for (int m = 0; m < NV; ++m) jac[m] = (jac[m] - rhs_sav[m]) / eps;
@@ -126,7 +127,8 @@ __device__ void kry_arnoldi(const Kin &K, const Thermo &th, double tol_iter_nr, 
 // One cell from time_start to time_end; returns the number of attempted steps.
 template <int NS>
 __device__ int krylov_cell(const Kin &K, const Thermo &th, double tol_iter_nr, double *vec, double t_iter,
-                           double dttry, double time_start, double time_end, int maxsteps, const double *fls) {
+                           double dttry, double time_start, double time_end, int maxsteps, const double *fls,
+                           double tau_k = 0.0) {
   constexpr int NV = NS + 4;
   using X = Idx<NS>;
   const double gam = 0.572816062482135;
@@ -145,7 +147,7 @@ __device__ int krylov_cell(const Kin &K, const Thermo &th, double tol_iter_nr, d
   while (time < (time_end - 1.0e-12)) {
     dt = dt < time_end - time ? dt : time_end - time;
     it = it + 1;
-    kry_arnoldi<NS>(K, th, tol_iter_nr, vec, dt, t_iter, fls, Vb, inv, rhs_sav);
+    kry_arnoldi<NS>(K, th, tol_iter_nr, vec, dt, t_iter, fls, Vb, inv, rhs_sav, tau_k);
     // A = Vb (I - inv)
     for (int kk = 0; kk < KM; ++kk)
       for (int ll = 0; ll < NV; ++ll) {
@@ -163,7 +165,7 @@ __device__ int krylov_cell(const Kin &K, const Thermo &th, double tol_iter_nr, d
       for (int m = 0; m < NV; ++m) ys[m] = vec[m];
       for (int jj = 0; jj < ii; ++jj)
         for (int m = 0; m < NV; ++m) ys[m] = ys[m] + dt * alpha[ii][jj] * Ks[m + NV * jj];
-      kry_rhs<NS>(K, th, tol_iter_nr, ys, t_iter, F);
+      kry_rhs<NS>(K, th, tol_iter_nr, ys, t_iter, F, tau_k);
       for (int m = 0; m < NV; ++m) GK[m] = 0.0;
       for (int jj = 0; jj < ii; ++jj)
         for (int m = 0; m < NV; ++m) GK[m] = GK[m] + gamc[ii][jj] / gam * Ks[m + NV * jj];
@@ -217,7 +219,7 @@ template <int NS>
 __global__ void __launch_bounds__(128) krylov_kernel(Box b, double *__restrict__ w, const double *__restrict__ w_aux,
                                                      Kin K, Thermo th, double tol_iter_nr, double dttry,
                                                      double time_start, double time_end, int maxsteps,
-                                                     const double *__restrict__ fl_sav) {
+                                                     const double *__restrict__ fl_sav, Pasr pasr) {
   using X = Idx<NS>;
   constexpr int NV = NS + 4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -233,8 +235,9 @@ __global__ void __launch_bounds__(128) krylov_kernel(Box b, double *__restrict__
   for (int m = 0; m < NV; ++m) vec[m] = w[pt + ps * m];
   if (fl_sav)
     for (int m = 0; m < NV; ++m) fls[m] = fl_sav[t + fs_ * m];
+  const double tau_k = pasr.on ? pasr.tau_k<NS>(w_aux, pt, ps, i, j, k) : 0.0;
   krylov_cell<NS>(K, th, tol_iter_nr, vec, w_aux[pt + ps * (X::J_T - 1)], dttry, time_start, time_end, maxsteps,
-                  fl_sav ? fls : nullptr);
+                  fl_sav ? fls : nullptr, tau_k);
   for (int m = 0; m < NV; ++m) w[pt + ps * m] = vec[m];
 }
 
@@ -254,9 +257,10 @@ extern "C" int streams_rosenbrock_krylov(int n_s, int nv, int nv_aux, int nx, in
                                          const double *fl_sav, int maxsteps, int enable_pasr, double les_c_yoshi,
                                          double les_c_mix, double les_c_eps, const double *dcsidx,
                                          const double *detady, const double *dzitdz, void *stream) {
-  (void)fl; (void)les_c_yoshi; (void)les_c_mix; (void)les_c_eps; (void)dcsidx; (void)detady; (void)dzitdz;
-  if (enable_pasr != 0) return STREAMS_E_OPTION;
+  (void)fl;
   if (nv != n_s + 4 || nv_aux < 2 * n_s + 13 || maxsteps < 0) return STREAMS_E_DIMS;
+  if (enable_pasr > 0 && (nv_aux < 2 * n_s + 14 || !dcsidx || !detady || !dzitdz)) return STREAMS_E_DIMS;
+  const Pasr pasr{enable_pasr > 0, les_c_yoshi, les_c_mix, les_c_eps, dcsidx, detady, dzitdz};
   if (simpler_splitting == 1 && fl_sav == nullptr) return STREAMS_E_DIMS;
   Box b{nx, ny, nz, ng};
   const long n = b.fstride();
@@ -266,6 +270,7 @@ extern "C" int streams_rosenbrock_krylov(int n_s, int nv, int nv_aux, int nx, in
   Thermo th{cv_coeff, nullptr, trange, nsetcv, indx_cp_l, indx_cp_r};
   const double *fls = simpler_splitting == 1 ? fl_sav : nullptr;
   STREAMS_DISPATCH_NS(n_s, (krylov_kernel<NS><<<blocks_for(n, 128), 128, 0, as_stream(stream)>>>(
-                                b, w, w_aux, K, th, tol_iter_nr, dttry, time_start, time_end, maxsteps, fls)));
+                                b, w, w_aux, K, th, tol_iter_nr, dttry, time_start, time_end, maxsteps, fls,
+                                pasr)));
   return hip_status();
 }

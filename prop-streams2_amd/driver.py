@@ -120,6 +120,8 @@ class Run:
             s.compute_dt(c.cfl)
         s.advance()
         s.compute_aux(central=1, ghost=0)
+        if s.num.enable_chemistry > 0 and self.time_from_last_write + s.dt >= c.dtsave:
+            s.compute_chem_aux()        # the diagnostics an output will write
         if (self.icyc - self.icyc0) % c.print_control == 0:
             self.compute_residual()
             if math.isnan(self.residual_rhou):
@@ -150,11 +152,14 @@ class Run:
         exchange, and the first dt."""
         s = self.s
         s.update_ghost(do_swap=1)
-        s.compute_aux()
-        jduc, jdiv = s.ns + 9, s.ns + 10
-        s.be.call("bcextr_var", s.nx, s.ny, s.nz, s.ng, s.w_aux[jdiv - 1], s.be.stream)
-        s.bcswap_var(jdiv)
-        s.bcswap_var(jduc)
+        if s.num.enable_les > 0:
+            s.compute_aux_les()
+        else:
+            s.compute_aux()
+            jduc, jdiv = s.ns + 9, s.ns + 10
+            s.be.call("bcextr_var", s.nx, s.ny, s.nz, s.ng, s.w_aux[jdiv - 1], s.be.stream)
+            s.bcswap_var(jdiv)
+            s.bcswap_var(jduc)
         s.compute_dt(self.c.cfl)
 
     # ------------------------------------------------------------------ restarts
@@ -245,6 +250,7 @@ class Run:
         ref, grid, bc = ini["ref_state"], ini["grid"], ini["bc"]
         ctl, num, flow, fluid = ini["controls"], ini["numerics"], ini["flow"], ini["fluid"]
         out = ini.get("output", {})
+        les = ini.get("lespar", {})
         mech = Mechanism(mechanism or str(pathlib.Path(path).parent / "input_cantera.yaml"))
         y = np.array([float(ref.get(f"init_y_{k + 1:02d}", 0.0)) for k in range(len(mech.species))])
         rho0 = float(ref["rho0"]) if "rho0" in ref else None
@@ -267,6 +273,15 @@ class Run:
             ros_maxsteps=_num(num, "ros_maxsteps", 10000, int), ros_maxtry=_num(num, "ros_maxtry", 40, int),
             rosenbrock_version=_num(num, "rosenbrock_version", 1, int),
             operator_splitting=_num(num, "operator_splitting", 2, int))
+        if _num(les, "enable_les", 0, int) > 0:
+            # multideal.F90:615-646: the constants the model needs are required keys
+            model = _num(les, "les_model", 0, int)
+            keys = ["les_pr", "les_sc", "les_c_wale"] + (["les_c_yoshi", "les_c_eps", "les_c_mix"] if model == 2 else [])
+            missing = [k for k in keys if k not in les]
+            if missing:
+                raise KeyError(f"[lespar] needs {', '.join(missing)}")
+            numerics = dataclasses.replace(numerics, enable_les=1, les_model=model,
+                                           **{k: float(les[k]) for k in keys})
         shape = [nx or _num(grid, "nxmax", 1, int), _num(grid, "nymax", 1, int), _num(grid, "nzmax", 1, int)]
         ng = _num(grid, "ng", 3, int)
         lengths = [_num(grid, "domain_size_x", 1.0), _num(grid, "domain_size_y", 1.0), _num(grid, "domain_size_z", 1.0)]

@@ -90,6 +90,20 @@ class Numerics:
     ros_maxtry: int = 40
     rosenbrock_version: int = 1      # 1: ROS4 (rosenbrock), 2: Rosenbrock-Krylov
     operator_splitting: int = 2      # with enable_chemistry == 2: 1 simpler (Krylov only), 2 Strang
+    # [lespar] (multideal.F90:615-646): enable_les with les_model 1 (WALE) or 2 (WALE +
+    # PaSR closure of the implicit chemistry); the constants have no defaults there
+    enable_les: int = 0
+    les_model: int = 1
+    les_pr: float = 0.0
+    les_sc: float = 0.0
+    les_c_wale: float = 0.0
+    les_c_yoshi: float = 0.0
+    les_c_eps: float = 0.0
+    les_c_mix: float = 0.0
+
+    @property
+    def enable_pasr(self) -> int:
+        return int(self.enable_les > 0 and self.les_model == 2)
 
 
 class Solver:
@@ -110,7 +124,16 @@ class Solver:
         self.ns = tables.n_species
         self.nv = self.ns + 4
         self.num = numerics or Numerics()
-        self.nv_aux = 2 * self.ns + 13 + (2 * (self.ns + 1) if self.num.enable_chemistry > 0 else 0)
+        num = self.num
+        if num.enable_les > 0:
+            if num.les_model not in (1, 2):
+                raise ValueError("LES model not implemented")
+            if num.les_model == 2 and num.enable_chemistry != 2:
+                raise ValueError("PaSR cannot be used without chemistry enabled or for explicit integration")
+        # nv_aux of multideal.F90:651-654: the eddy viscosity slot (J_LES1), the chemistry
+        # diagnostics and the PaSR factor come after the 2 N_S + 13 base variables
+        self.nv_aux = (2 * self.ns + 13 + (1 if num.enable_les > 0 else 0)
+                       + (2 * (self.ns + 1) if num.enable_chemistry > 0 else 0) + num.enable_pasr)
         self.nx, self.ny, self.nz, self.ng = nx, ny, nz, ng
         self.bctags = list(bctags)
         self.periodic = np.array([int(bctags[0] == 0 and bctags[1] == 0), int(bctags[2] == 0 and bctags[3] == 0),
@@ -234,8 +257,45 @@ class Solver:
                      self.w_aux, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], self.eps_sensor,
                      self.num.sensor_type, self.be.stream)
 
+    def compute_aux_les(self):
+        """compute_aux_les (multideal_gpu.F90:445-498): Y, rho and velocity, the WALE eddy
+        viscosity with the sensor, their ghost layers, then the thermodynamic state and the
+        transport with the eddy viscosity added."""
+        n, be = self.num, self.be
+        tb = self.tb
+        nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
+        be.call("eval_velaux", self.ns, nx, ny, nz, ng, self.nv, self.nv_aux, self.w, self.w_aux, be.stream)
+        be.call("les_wale_mut", self.ns, nx, ny, nz, ng, self.nv_aux, n.ep_order, self.w_aux, self.cd1, self.dxi[0],
+                self.dxi[1], self.dxi[2], n.les_c_wale, self.eps_sensor, n.sensor_type, be.stream)
+        jduc, jdiv, jles = self.ns + 9, self.ns + 10, 2 * self.ns + 14
+        self.bcswap_var(jduc)
+        be.call("bcextr_var", nx, ny, nz, ng, self.w_aux[jdiv - 1], be.stream)
+        self.bcswap_var(jdiv)
+        be.call("bcextr_var", nx, ny, nz, ng, self.w_aux[jles - 1], be.stream)
+        self.bcswap_var(jles)
+        be.call("eval_aux_les", self.ns, nx, ny, nz, ng, self.nv, self.nv_aux, self.w, self.w_aux, self.p0,
+                tb.t_min_tab, tb.dt_tab, R_UNIV, self.t_rgas, self.t_cv, self.t_cp, tb.nsetcv, self.t_trange,
+                tb.indx_cp_l, tb.indx_cp_r, TOL_ITER_NR, self.t_mw, self.t_mwinv, self.t_visc, self.t_lambda,
+                self.t_diff, tb.num_t_tab, n.les_pr, n.les_sc, be.stream)
+
+    def compute_chem_aux(self):
+        """compute_chem_aux (multideal_gpu.F90:500-534): production rates and heat release
+        rate into w_aux (the output-time diagnostics)."""
+        n, tb = self.num, self.tb
+        if getattr(self, "t_h298", None) is None:
+            self.t_h298 = self.be.asarray(tb.h298)
+        self.be.call("eval_chem_aux", self.ns, self.nx, self.ny, self.nz, self.nv_aux, self.ng, self.nr, self.w_aux,
+                     self.t_mw, self.t_arr_a, self.t_arr_b, self.t_arr_ea, self.t_tb, self.t_fall, self.t_rty,
+                     self.t_rev, self.t_rco, self.t_pco, self.t_kc, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV,
+                     self.dxi[0], self.dxi[1], self.dxi[2], self.t_h298, n.enable_pasr, n.les_c_yoshi, n.les_c_mix,
+                     n.les_c_eps, self.be.stream)
+
     def compute_aux(self, central: int = 1, ghost: int = 1):
-        """compute_aux (multideal_gpu.F90:361-443)."""
+        """compute_aux (multideal_gpu.F90:361-443); with LES every call site of the
+        reference calls compute_aux_les instead, whatever the box."""
+        if self.num.enable_les > 0:
+            self.compute_aux_les()
+            return
         nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
         if central == 1 and ghost == 1:
             self.eval_aux()
@@ -310,7 +370,8 @@ class Solver:
                      self.t_fall, self.t_rty, self.t_rev, self.t_rco, self.t_pco, self.t_kc, tb.num_t_tab,
                      tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l,
                      tb.indx_cp_r, dt_chem, self.time, self.time + dt_chem, n.ros_maxsteps, n.ros_maxtry, n.ros_eps,
-                     0, 0, 0.0, 0.0, 0.0, self.dxi[0], self.dxi[1], self.dxi[2], self.be.stream)
+                     0, n.enable_pasr, n.les_c_yoshi, n.les_c_mix, n.les_c_eps, self.dxi[0], self.dxi[1], self.dxi[2],
+                     self.be.stream)
 
     def rosenbrock_krylov(self, dt_chem: float, fl_sav=None):
         """rosenbrock_krylov (multideal_gpu.F90:769-838); fl_sav given: simpler splitting,
@@ -321,7 +382,8 @@ class Solver:
                      self.t_tb, self.t_fall, self.t_rty, self.t_rev, self.t_rco, self.t_pco, self.t_kc, tb.num_t_tab,
                      tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l,
                      tb.indx_cp_r, dt_chem, self.time, self.time + dt_chem, int(fl_sav is not None), fl_sav,
-                     n.ros_maxsteps, 0, 0.0, 0.0, 0.0, self.dxi[0], self.dxi[1], self.dxi[2], self.be.stream)
+                     n.ros_maxsteps, n.enable_pasr, n.les_c_yoshi, n.les_c_mix, n.les_c_eps, self.dxi[0], self.dxi[1],
+                     self.dxi[2], self.be.stream)
 
     def update_ghost(self, do_swap: int = 0):
         """update_ghost (multideal_gpu.F90:1165-1238): the physical boundary conditions of
@@ -382,11 +444,14 @@ class Solver:
         """bcswap, compute_aux, sensor exchange, euler_x/y/z and visflx into fl."""
         n, be = self.num, self.be
         self.bcswap()
-        self.compute_aux()
-        jduc, jdiv = self.ns + 9, self.ns + 10
-        be.call("bcextr_var", self.nx, self.ny, self.nz, self.ng, self.w_aux[jdiv - 1], be.stream)
-        self.bcswap_var(jduc)
-        self.bcswap_var(jdiv)
+        if n.enable_les > 0:
+            self.compute_aux_les()
+        else:
+            self.compute_aux()
+            jduc, jdiv = self.ns + 9, self.ns + 10
+            be.call("bcextr_var", self.nx, self.ny, self.nz, self.ng, self.w_aux[jdiv - 1], be.stream)
+            self.bcswap_var(jduc)
+            self.bcswap_var(jdiv)
         self.euler(0)
         if self.ny > 1:
             self.euler(1)
@@ -413,9 +478,10 @@ class Solver:
 
     def fused_supported(self) -> bool:
         """The fused substep covers full-range sweeps (no non-reflecting offsets), the
-        conservative viscous form; explicit chemistry is folded into the tail, implicit
-        chemistry (enable_chemistry = 2) runs around the substeps."""
+        conservative viscous form without LES; explicit chemistry is folded into the tail,
+        implicit chemistry (enable_chemistry = 2) runs around the substeps."""
         return (self.eul == (1, self.nx, 1, self.ny, 1, self.nz) and self.num.conservative_viscous == 1
+                and self.num.enable_les == 0
                 and self.be.has("rhs_sweep") and self.be.has("rhs_finish"))
 
     def rk_substep_fused(self, istep: int):
@@ -464,7 +530,7 @@ class Solver:
                 self.rk_substep(istep, simpler=simpler)
 
     def advance(self, fused: Optional[bool] = None):
-        """advance_solution (multideal_gpu.F90:1853-1931) without LES / energy deposition."""
+        """advance_solution (multideal_gpu.F90:1853-1931) without energy deposition."""
         n = self.num
         if n.enable_chemistry == 2 and n.operator_splitting == 1:
             if n.rosenbrock_version != 2:
