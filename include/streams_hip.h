@@ -316,6 +316,19 @@ int streams_compute_dt(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_a
                        int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
                        const double *trange, const int *fluid_mask, void *stream);
 
+/* No counterpart in the reference: streams_compute_dt for callers that cannot allocate
+ * device memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; the partial maxima live in a fixed device array
+ * of the library (147,528 bytes per GPU, 9 doubles for each of at most 2048 blocks plus the
+ * 9 results, held until the process ends) instead of stream-ordered scratch.  Calls that
+ * overlap on one GPU share that array and race.  Any box size is accepted (the grid is
+ * capped at 2048 blocks); nv_aux below 2 n_s + 13 returns STREAMS_E_DIMS. */
+int streams_compute_dt_noalloc(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux,
+                               const double *dcsidx, const double *detady, const double *dzitdz,
+                               const double *dcsidxs, const double *detadys, const double *dzitdzs,
+                               const double *w, const double *w_aux, double *dtmax9,
+                               int indx_cp_l, int indx_cp_r, const double *cp_coeff, int nsetcv,
+                               const double *trange, const int *fluid_mask, void *stream);
 
 
 /* ---- host <-> device field copies and checkpoints (base_gpu.F90:1266-1325) ---------- */

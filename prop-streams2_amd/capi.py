@@ -53,6 +53,7 @@ _ARGS = {
     "update_simpler_flux": "iiii" "ppp" "d" "s",
     "update_field": "iiiii" "ppp" "s",
     "compute_dt": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
+    "compute_dt_noalloc": "iiiiiii" "pppppp" "pp" "P" "ii" "p" "i" "pp" "s",
     "copy_cpu_gpu": "iiiii" "P" "p" "s",
     "copy_gpu_cpu": "iiiii" "p" "P" "s",
     "field_interior": "iiiii" "pp" "i" "s",

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) compute_dt_kernel(Box b, const double *__
     for (int q = 0; q < 9; ++q) partial[(long)blockIdx.x * 9 + q] = red[q][0];
 }
 
-__global__ void __launch_bounds__(256) reduce9_kernel(int nblk, const double *__restrict__ partial,
+static __global__ void __launch_bounds__(256) reduce9_kernel(int nblk, const double *__restrict__ partial,
                                                       double *__restrict__ out) {
   __shared__ double red[9][256];
   double mx[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
