@@ -309,6 +309,8 @@ def main():
                     help="reactor server on a Poisson stream of this many seconds (0: skip)")
     ap.add_argument("--c4", action="store_true",
                     help="also time the 512x256x128 14-species syngas mixing layer with ~10 %% WENO faces")
+    ap.add_argument("--reactors-only", action="store_true",
+                    help="only the batched-chemistry part (for profiling the reactor kernel)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (5 times): the known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -323,6 +325,9 @@ def main():
 
     pkg = _pkg()
     import torch
+    if args.reactors_only:
+        print(json.dumps({"batched_chemistry": reactor_bench(pkg, args.reactors, args.reactors_fp64)}))
+        return
     tables = pkg.cases.h2_air_mechanism().tables()
     n = args.n
     print(f"[bench] building {n}^3 input", file=sys.stderr, flush=True)

@@ -48,3 +48,84 @@ def test_fortran_dropin_runs_on_gpu():
     res = subprocess.run([str(EXE)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "dropin_check: ok" in res.stdout
+
+
+
+def _case_blocks(s, w, tg):
+    """The case's arrays in the program's block order (host numpy, the layouts the solver
+    holds on the device: Fortran order, reversed axes)."""
+    import numpy as np
+    h = s.be.to_numpy
+    waux = np.zeros(s.w_aux.shape)
+    waux[s.ns + 4] = tg
+    return [w, waux, h(s.t_rgas), h(s.t_cv), h(s.t_cp), h(s.t_trange), h(s.t_mw), h(s.t_mwinv), h(s.t_visc),
+            h(s.t_lambda), h(s.t_diff), h(s.t_eoi), h(s.t_nainsp), h(s.t_beta0), h(s.cd1), h(s.dxi[0]),
+            h(s.dxi[1]), h(s.dxi[2]), h(s.t_x), h(s.t_epo), h(s.t_arr_a), h(s.t_arr_b), h(s.t_arr_ea), h(s.t_tb),
+            h(s.t_fall), h(s.t_rty), h(s.t_rev), h(s.t_rco), h(s.t_pco), h(s.t_kc), np.zeros(s.fl.shape),
+            np.zeros(s.fhat.shape)]
+
+
+@pytest.mark.gpu
+def test_fortran_module_matches_the_c_abi_bit_for_bit(pkg, h2_tables, tmp_path):
+    """eval_aux_cuf, eval_aux2_cuf, euler_x (WENO on part of the faces), visflx_x_cuf,
+    compute_chemistry_cuf and rosenbrock_krylov called through the Fortran module with the
+    reference's argument lists, against the same calls made through ctypes on the same
+    inputs: w_aux, fhat, fl and w bit for bit."""
+    import numpy as np
+    from parity_util import box_state
+    assert EXE.exists(), f"{EXE} missing: build with prop-streams2_amd/build.py (needs amdflang)"
+    from prop_streams2_amd.solver import R_UNIV, TOL_ITER_NR
+    hip = pkg.solver.HipBackend()
+    shape, ng = (12, 10, 8), 3
+    w, tg, info = box_state(pkg, h2_tables, shape, seed=12, temperature=1400.0, perturb_y=0.3)
+    num = pkg.solver.Numerics(enable_chemistry=2, rosenbrock_version=2, weno_scheme=3, sensor_threshold=0.1,
+                              sensor_type=1)
+    s = pkg.solver.Solver(hip, h2_tables, *shape, ng, [info["length"]] * 3, numerics=num, rho0=info["rho0"],
+                          u0=info["u_rms"])
+    tb = h2_tables
+    lmax, nkeep, dt_chem, maxsteps = 3, 0, 2e-7, 10000
+    blocks = _case_blocks(s, w, tg)
+    case, out = tmp_path / "case.bin", tmp_path / "out.bin"
+    with open(case, "wb") as fh:
+        fh.write(np.array([s.nx, s.ny, s.nz, s.ng, s.ns, s.nv, s.nv_aux, tb.nsetcv, tb.indx_cp_l, tb.indx_cp_r,
+                           tb.num_t_tab, s.nr, lmax, nkeep, num.weno_scheme, maxsteps], dtype="<i4").tobytes())
+        hd = np.zeros(16)
+        hd[:10] = [s.p0, tb.t_min_tab, tb.dt_tab, R_UNIV, TOL_ITER_NR, s.rho0, s.u0, num.sensor_threshold,
+                   s.eps_sensor, dt_chem]
+        fh.write(hd.astype("<f8").tobytes())
+        for a in blocks:
+            a = np.ascontiguousarray(a)
+            kind = 2 if a.dtype == np.int32 else 1
+            fh.write(np.array([kind], dtype="<i4").tobytes() + np.array([a.size], dtype="<i8").tobytes())
+            fh.write(a.astype("<i4" if kind == 2 else "<f8").tobytes())
+    print("[dropin] Fortran case run", flush=True)
+    res = subprocess.run([str(EXE), str(case), str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(out, dtype="<f8")
+    sizes = [blocks[1].size, blocks[31].size, blocks[30].size, blocks[0].size]
+    assert raw.size == sum(sizes)
+    f_waux, f_fhat, f_fl, f_w = np.split(raw, np.cumsum(sizes)[:-1])
+    # the same sequence through ctypes
+    be = hip
+    s.set_state(w, tg)
+    be.copy_(s.fl, be.zeros(s.fl.shape))
+    be.copy_(s.fhat, be.zeros(s.fhat.shape))
+    s.eval_aux()
+    s.be.call("eval_aux2", s.ns, s.nx, s.ny, s.nz, s.ng, s.nv_aux, 2 * lmax, s.w_aux, s.cd1, s.dxi[0], s.dxi[1],
+              s.dxi[2], s.eps_sensor, 1, be.stream)
+    c_waux = be.to_numpy(s.w_aux).copy()
+    s.euler(0)
+    be.call("visflx_x", s.ns, s.nx, s.ny, s.nz, s.nv, s.nv_aux, s.ng, s.t_x, s.w_aux, s.fl, s.fhat, tb.indx_cp_l,
+            tb.indx_cp_r, s.t_cp, tb.nsetcv, s.t_trange, s.t_rgas, R_UNIV, be.stream)
+    c_fhat = be.to_numpy(s.fhat).copy()
+    s.compute_chemistry()
+    c_fl = be.to_numpy(s.fl).copy()
+    s.time = 0.0
+    s.rosenbrock_krylov(dt_chem)
+    be.sync()
+    c_w = be.to_numpy(s.w).copy()
+    weno = (c_waux[s.ns + 8] > num.sensor_threshold).mean()
+    print(f"[dropin] sensor above threshold on {weno:.2%} of the points", flush=True)
+    for name, f, c in (("w_aux", f_waux, c_waux), ("fhat", f_fhat, c_fhat), ("fl", f_fl, c_fl), ("w", f_w, c_w)):
+        assert np.array_equal(f, c.ravel()), name
+    assert np.abs(c_w - w).max() > 0.0          # the chemistry step moved the state
