@@ -6,7 +6,6 @@
 // one kernel per axis with the two sides and all variables in one grid.  When an axis is
 // shorter than ng the source slab overlaps the ghost slab of the same axis; that case is
 // staged through a device buffer so that every source is read before any ghost is written.
-#include <mutex>
 
 #include "common.h"
 
@@ -73,23 +72,6 @@ __global__ void __launch_bounds__(256) swap_unpack_kernel(Box b, Slab s, long n,
   long src, dst;
   slab_src_dst(b, s.ax, side, a, bq, c, src, dst);
   w[dst + b.pstride() * m] = buf[t];
-}
-
-// ---- staging buffer for the short-axis case (library-owned, grown on demand)
-static std::mutex g_stage_mutex;
-static double *g_stage = nullptr;
-static size_t g_stage_bytes = 0;
-
-static double *stage_buffer(size_t bytes) {
-  std::lock_guard<std::mutex> lk(g_stage_mutex);
-  if (bytes > g_stage_bytes) {
-    if (g_stage) (void)hipFree(g_stage);
-    g_stage = nullptr;
-    g_stage_bytes = 0;
-    if (hipMalloc(&g_stage, bytes) != hipSuccess) return nullptr;
-    g_stage_bytes = bytes;
-  }
-  return g_stage;
 }
 
 // bcextr_var_cuf / bcextr_cuf: zero-order extrapolation of face ilat (1..6)
@@ -160,10 +142,15 @@ int swap_axis(const Box &b, int ax, int nv, double *w, hipStream_t st) {
   if (n_ax >= b.ng) {
     swap_direct_kernel<<<blocks_for(n, 256), 256, 0, st>>>(b, s, n, w);
   } else {
-    double *buf = stage_buffer((size_t)n * sizeof(double));
-    if (!buf) return STREAMS_E_NOMEM;
+    // stream-ordered staging: calls on different streams each get their own buffer
+    double *buf = nullptr;
+    if (hipMallocAsync((void **)&buf, (size_t)n * sizeof(double), st) != hipSuccess) return STREAMS_E_NOMEM;
     swap_pack_kernel<<<blocks_for(n, 256), 256, 0, st>>>(b, s, n, w, buf);
     swap_unpack_kernel<<<blocks_for(n, 256), 256, 0, st>>>(b, s, n, w, buf);
+    const int rc = hip_status();
+    const hipError_t e = hipFreeAsync(buf, st);
+    if (rc) return rc;
+    return e == hipSuccess ? 0 : (int)e;
   }
   return hip_status();
 }

@@ -42,6 +42,28 @@ class ReactorBatchGPU:
         self.t_cv = f(tables.cv_coeff)
         self.t_trange = f(tables.trange)
         self.work = backend.zeros((2,), "i4")
+        self.wg_per_cu = 0           # 0: the library's default (4); set by tune()
+        self.tuning = {}
+
+    def tune(self, w, tg, dttry, time_end, mixed=1, candidates=(1, 2, 4, 8), pressure=None) -> int:
+        """Pick the persistent grid's workgroups per CU for batches like this one: one timed
+        call per candidate on device copies of the inputs, the fastest kept (and used by
+        later calls that pass wg_per_cu = 0).  The results do not depend on the choice:
+        lanes pull reactors from the shared counter."""
+        torch = self.be.torch
+        times = {}
+        for per in candidates:
+            wc, tc = w.clone(), tg.clone()
+            st = torch.cuda.current_stream(self.be.device)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            self.integrate(wc, tc, dttry, time_end, wg_per_cu=per, mixed=mixed, pressure=pressure)
+            e1.record(st)
+            e1.synchronize()
+            times[per] = e0.elapsed_time(e1) / 1e3
+        self.wg_per_cu = min(times, key=times.get)
+        self.tuning = {"wg_per_cu": self.wg_per_cu, "seconds": times, "reactors": int(w.shape[1])}
+        return self.wg_per_cu
 
     def integrate(self, w, tg, dttry, time_end, nacc=None, nrej=None, wg_per_cu: int = 0, stream=None,
                   mixed=False, pressure=None, work=None, t_done=None, dt_next=None,
@@ -60,7 +82,7 @@ class ReactorBatchGPU:
                      self.t_cp, tb.nreactions, self.t_kin, self.t_kc, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV,
                      TOL_ITER_NR, self.t_cv, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r,
                      self.maxsteps if maxsteps is None else maxsteps, self.maxtry, self.tol, nacc, nrej, t_done,
-                     dt_next, self.work if work is None else work, wg_per_cu, int(mixed),
+                     dt_next, self.work if work is None else work, wg_per_cu or self.wg_per_cu, int(mixed),
                      self.be.stream if stream is None else stream)
 
 
