@@ -22,7 +22,7 @@ namespace streams {
 
 // ---------------------------------------------------------------- shock pre-pass
 // one thread per face of every line; shocked faces get their WENO flux in fhat
-template <int NS, int DIR>
+template <int NS, int DIR, bool RUS>
 __global__ void __launch_bounds__(128) weno_prepass_kernel(Box b, EulerArgs A, int t1n, long nfaces) {
   using X = Idx<NS>;
   constexpr int NV = NS + 4;
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(128) weno_prepass_kernel(Box b, EulerArgs A, i
   int ord = A.weno_scheme + epo_at<DIR>(b, A.epo, t1, t2, f);
   ord = ord < 1 ? 1 : ord;
   double fh[NV];
-  weno_flux<NS, DIR>(L, f, A, ord, fh);
+  weno_flux<NS, DIR, RUS>(L, f, A, ord, fh);
   long fb, fst;
   line_base<DIR>(b, t1, t2, fb, fst);
 #pragma unroll
@@ -124,15 +124,23 @@ __global__ void __launch_bounds__(128) euler_yz_fused_kernel(Box b, EulerArgs A,
   }
 }
 
+template <int NS, int DIR>
+inline void prepass_launch(const Box &b, const EulerArgs &A, int t1n, long nfaces, hipStream_t st) {
+  if (A.flux_splitting == 1)
+    weno_prepass_kernel<NS, DIR, true><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces);
+  else
+    weno_prepass_kernel<NS, DIR, false><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces);
+}
+
 int weno_prepass_launch(int dir, int ns, const Box &b, const EulerArgs &A, hipStream_t st) {
   const int nf = A.fe - A.fs + 1;
   const int t1n = dir == 1 ? b.ny : b.nx;
   const int t2n = dir == 3 ? b.ny : b.nz;
   const long nfaces = (long)nf * t1n * t2n;
   if (nfaces <= 0) return 0;
-  if (dir == 1) { STREAMS_DISPATCH_NS(ns, (weno_prepass_kernel<NS, 1><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces))); }
-  else if (dir == 2) { STREAMS_DISPATCH_NS(ns, (weno_prepass_kernel<NS, 2><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces))); }
-  else { STREAMS_DISPATCH_NS(ns, (weno_prepass_kernel<NS, 3><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces))); }
+  if (dir == 1) { STREAMS_DISPATCH_NS(ns, (prepass_launch<NS, 1>(b, A, t1n, nfaces, st))); }
+  else if (dir == 2) { STREAMS_DISPATCH_NS(ns, (prepass_launch<NS, 2>(b, A, t1n, nfaces, st))); }
+  else { STREAMS_DISPATCH_NS(ns, (prepass_launch<NS, 3>(b, A, t1n, nfaces, st))); }
   return hip_status();
 }
 
@@ -145,7 +153,7 @@ int euler_launch(const Box &b, EulerArgs A, hipStream_t st) {
   const int t2n = DIR == 3 ? b.ny : b.nz;
   const long nfaces = (long)nf * t1n * t2n;
   if (check) {
-    weno_prepass_kernel<NS, DIR><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces);
+    prepass_launch<NS, DIR>(b, A, t1n, nfaces, st);
     int rc = hip_status();
     if (rc) return rc;
   }

@@ -152,6 +152,146 @@ __device__ void keep_flux(const LineT &L, int f, int lmax, int nkeep, const doub
 }
 
 // ---------------------------------------------------------------------------- WENO
+// One field of wenorec_1d (kernels_gpu.F90:2353-2637): vp[k + 3] and vm[k + 3] hold the
+// plus / minus split flux at stencil offset k (the cell f + k of face f, k = -3..4), bsc
+// the field's smoothness-indicator scale.  With offsets fixed at compile time the stencil
+// lives in registers.
+__device__ inline double wenorec1(const double *vp, const double *vm, double bsc, int ord, int version) {
+#define VP(k) vp[(k) + 3]
+#define VM(k) vm[(k) + 3]
+  if (ord == 1) {
+    return VP(0) + VM(+1);
+  } else if (ord == 2) {
+    double bp0 = (VP(0) - VP(-1)) * (VP(0) - VP(-1));
+    double bp1 = (VP(+1) - VP(0)) * (VP(+1) - VP(0));
+    bp1 = bsc * bp1; bp0 = bsc * bp0;
+    double bm0 = (VM(+2) - VM(+1)) * (VM(+2) - VM(+1));
+    double bm1 = (VM(+1) - VM(0)) * (VM(+1) - VM(0));
+    bm1 = bsc * bm1; bm0 = bsc * bm0;
+    const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
+    double t;
+    t = 0.000001 + bp0; bp0 = d0 / (t * t);
+    t = 0.000001 + bm0; bm0 = d0 / (t * t);
+    t = 0.000001 + bp1; bp1 = d1 / (t * t);
+    t = 0.000001 + bm1; bm1 = d1 / (t * t);
+    double sump = bp0 + bp1, summ = bm0 + bm1;
+    bp0 = bp0 / sump; bp1 = bp1 / sump; bm0 = bm0 / summ; bm1 = bm1 / summ;
+    double vminus = bp0 * (-VP(-1) + 3 * VP(0)) + bp1 * (VP(0) + VP(+1));
+    double vplus = bm0 * (-VM(+2) + 3 * VM(+1)) + bm1 * (VM(0) + VM(+1));
+    return 0.5 * (vminus + vplus);
+  } else if (ord == 3) {
+    const double d0 = 13.0 / 12.0, d1 = 1.0 / 4.0;
+    const double c0 = 1.0 / 3.0, c1 = 5.0 / 6.0, c2 = -1.0 / 6.0, c3 = -7.0 / 6.0, c4 = 11.0 / 6.0;
+    const double dw0 = 1.0 / 10.0, dw1 = 6.0 / 10.0, dw2 = 3.0 / 10.0;
+    double a, bq;
+    a = VP(0) - 2.0 * VP(+1) + VP(+2); bq = 3.0 * VP(0) - 4.0 * VP(+1) + VP(+2);
+    double bp2 = d0 * a * a + d1 * bq * bq;
+    a = VP(-1) - 2.0 * VP(0) + VP(+1); bq = VP(-1) - VP(+1);
+    double bp1 = d0 * a * a + d1 * bq * bq;
+    a = VP(0) - 2.0 * VP(-1) + VP(-2); bq = 3.0 * VP(0) - 4.0 * VP(-1) + VP(-2);
+    double bp0 = d0 * a * a + d1 * bq * bq;
+    bp2 = bsc * bp2; bp1 = bsc * bp1; bp0 = bsc * bp0;
+    a = VM(+1) - 2.0 * VM(0) + VM(-1); bq = 3.0 * VM(+1) - 4.0 * VM(0) + VM(-1);
+    double bm2 = d0 * a * a + d1 * bq * bq;
+    a = VM(+2) - 2.0 * VM(+1) + VM(0); bq = VM(+2) - VM(0);
+    double bm1 = d0 * a * a + d1 * bq * bq;
+    a = VM(+1) - 2.0 * VM(+2) + VM(+3); bq = 3.0 * VM(+1) - 4.0 * VM(+2) + VM(+3);
+    double bm0 = d0 * a * a + d1 * bq * bq;
+    bm2 = bsc * bm2; bm1 = bsc * bm1; bm0 = bsc * bm0;
+    if (version == 0) {
+      double t;
+      t = 0.000001 + bp0; bp0 = dw0 / (t * t);
+      t = 0.000001 + bm0; bm0 = dw0 / (t * t);
+      t = 0.000001 + bp1; bp1 = dw1 / (t * t);
+      t = 0.000001 + bm1; bm1 = dw1 / (t * t);
+      t = 0.000001 + bp2; bp2 = dw2 / (t * t);
+      t = 0.000001 + bm2; bm2 = dw2 / (t * t);
+    } else {
+      const double eps40 = 1.0e-40;
+      double tau5p = fabs(bp0 - bp2) + eps40, tau5m = fabs(bm0 - bm2) + eps40;
+      bp0 = (bp0 + eps40) / (bp0 + tau5p); bm0 = (bm0 + eps40) / (bm0 + tau5m);
+      bp1 = (bp1 + eps40) / (bp1 + tau5p); bm1 = (bm1 + eps40) / (bm1 + tau5m);
+      bp2 = (bp2 + eps40) / (bp2 + tau5p); bm2 = (bm2 + eps40) / (bm2 + tau5m);
+      bp0 = dw0 / bp0; bm0 = dw0 / bm0;
+      bp1 = dw1 / bp1; bm1 = dw1 / bm1;
+      bp2 = dw2 / bp2; bm2 = dw2 / bm2;
+    }
+    double sump = bp0 + bp1 + bp2, summ = bm0 + bm1 + bm2;
+    bp0 = bp0 / sump; bp1 = bp1 / sump; bp2 = bp2 / sump;
+    bm0 = bm0 / summ; bm1 = bm1 / summ; bm2 = bm2 / summ;
+    double vminus = bp2 * (c0 * VP(0) + c1 * VP(+1) + c2 * VP(+2)) +
+                    bp1 * (c2 * VP(-1) + c1 * VP(0) + c0 * VP(+1)) +
+                    bp0 * (c0 * VP(-2) + c3 * VP(-1) + c4 * VP(0));
+    double vplus = bm2 * (c0 * VM(+1) + c1 * VM(0) + c2 * VM(-1)) +
+                   bm1 * (c2 * VM(+2) + c1 * VM(+1) + c0 * VM(0)) +
+                   bm0 * (c0 * VM(+3) + c3 * VM(+2) + c4 * VM(+1));
+    return vminus + vplus;
+  } else {  // ord == 4, WENO-7
+    const double d1 = 1.0 / 36.0, d2 = 13.0 / 12.0, d3 = 781.0 / 720.0;
+    const double dw[4] = {1.0 / 35.0, 12.0 / 35.0, 18.0 / 35.0, 4.0 / 35.0};
+    double bp[4], bm[4], a, bq, c;
+    a = -11 * VP(0) + 18 * VP(+1) - 9 * VP(+2) + 2 * VP(+3);
+    bq = 2 * VP(0) - 5 * VP(+1) + 4 * VP(+2) - VP(+3);
+    c = -VP(0) + 3 * VP(+1) - 3 * VP(+2) + VP(+3);
+    bp[3] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = -2 * VP(-1) - 3 * VP(0) + 6 * VP(+1) - VP(+2);
+    bq = VP(-1) - 2 * VP(0) + VP(+1);
+    c = -VP(-1) + 3 * VP(0) - 3 * VP(+1) + VP(+2);
+    bp[2] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = VP(-2) - 6 * VP(-1) + 3 * VP(0) + 2 * VP(+1);
+    bq = VP(-1) - 2 * VP(0) + VP(+1);
+    c = -VP(-2) + 3 * VP(-1) - 3 * VP(0) + VP(+1);
+    bp[1] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = -2 * VP(-3) + 9 * VP(-2) - 18 * VP(-1) + 11 * VP(0);
+    bq = -VP(-3) + 4 * VP(-2) - 5 * VP(-1) + 2 * VP(0);
+    c = -VP(-3) + 3 * VP(-2) - 3 * VP(-1) + VP(0);
+    bp[0] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = -11 * VM(+1) + 18 * VM(0) - 9 * VM(-1) + 2 * VM(-2);
+    bq = 2 * VM(+1) - 5 * VM(0) + 4 * VM(-1) - VM(-2);
+    c = -VM(+1) + 3 * VM(0) - 3 * VM(-1) + VM(-2);
+    bm[3] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = -2 * VM(+2) - 3 * VM(+1) + 6 * VM(0) - VM(-1);
+    bq = VM(+2) - 2 * VM(+1) + VM(0);
+    c = -VM(+2) + 3 * VM(+1) - 3 * VM(0) + VM(-1);
+    bm[2] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = VM(+3) - 6 * VM(+2) + 3 * VM(+1) + 2 * VM(0);
+    bq = VM(+2) - 2 * VM(+1) + VM(0);
+    c = -VM(+3) + 3 * VM(+2) - 3 * VM(+1) + VM(0);
+    bm[1] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    a = -2 * VM(+4) + 9 * VM(+3) - 18 * VM(+2) + 11 * VM(+1);
+    bq = -VM(+4) + 4 * VM(+3) - 5 * VM(+2) + 2 * VM(+1);
+    c = -VM(+4) + 3 * VM(+3) - 3 * VM(+2) + VM(+1);
+    bm[0] = d1 * a * a + d2 * bq * bq + d3 * c * c;
+    double sump = 0, summ = 0;
+    for (int l = 0; l < 4; ++l) {
+      bp[l] = bsc * bp[l];
+      bm[l] = bsc * bm[l];
+    }
+    for (int l = 0; l < 4; ++l) {
+      double t = 0.000001 + bp[l];
+      bp[l] = dw[l] / (t * t);
+      t = 0.000001 + bm[l];
+      bm[l] = dw[l] / (t * t);
+      sump = sump + bp[l];
+      summ = summ + bm[l];
+    }
+    for (int l = 0; l < 4; ++l) { bp[l] = bp[l] / sump; bm[l] = bm[l] / summ; }
+    double vminus = bp[3] * (6 * VP(0) + 26 * VP(+1) - 10 * VP(+2) + 2 * VP(+3)) +
+                    bp[2] * (-2 * VP(-1) + 14 * VP(0) + 14 * VP(+1) - 2 * VP(+2)) +
+                    bp[1] * (2 * VP(-2) - 10 * VP(-1) + 26 * VP(0) + 6 * VP(+1)) +
+                    bp[0] * (-6 * VP(-3) + 26 * VP(-2) - 46 * VP(-1) + 50 * VP(0));
+    double vplus = bm[3] * (6 * VM(+1) + 26 * VM(0) - 10 * VM(-1) + 2 * VM(-2)) +
+                   bm[2] * (-2 * VM(+2) + 14 * VM(+1) + 14 * VM(0) - 2 * VM(-1)) +
+                   bm[1] * (2 * VM(+3) - 10 * VM(+2) + 26 * VM(+1) + 6 * VM(0)) +
+                   bm[0] * (-6 * VM(+4) + 26 * VM(+3) - 46 * VM(+2) + 50 * VM(+1));
+    return (vminus + vplus) / 24.0;
+  }
+#undef VP
+#undef VM
+}
+
+// wenorec over all NV fields of vp / vm (NV x 8, column-major by stencil index l = 1..8,
+// cell f + l - iweno)
 template <int NS>
 __device__ void wenorec(const double *vp, const double *vm, double *vhat, int iweno, int ord, int version,
                         double rho0, double u0, bool rusanov) {
@@ -171,152 +311,29 @@ __device__ void wenorec(const double *vp, const double *vm, double *vhat, int iw
     bsc[NS] = r2u4i; bsc[NS + 1] = r2u4i; bsc[NS + 2] = r2u4i;
     bsc[NS + 3] = r2u4i / u0_2;
   }
-#define VP(m, l) vp[(m) + NV * ((l) - 1)]
-#define VM(m, l) vm[(m) + NV * ((l) - 1)]
-  const int i = iweno;
   for (int m = 0; m < NV; ++m) {
-    if (ord == 1) {
-      vhat[m] = VP(m, i) + VM(m, i + 1);
-    } else if (ord == 2) {
-      double bp0 = (VP(m, i) - VP(m, i - 1)) * (VP(m, i) - VP(m, i - 1));
-      double bp1 = (VP(m, i + 1) - VP(m, i)) * (VP(m, i + 1) - VP(m, i));
-      bp1 = bsc[m] * bp1; bp0 = bsc[m] * bp0;
-      double bm0 = (VM(m, i + 2) - VM(m, i + 1)) * (VM(m, i + 2) - VM(m, i + 1));
-      double bm1 = (VM(m, i + 1) - VM(m, i)) * (VM(m, i + 1) - VM(m, i));
-      bm1 = bsc[m] * bm1; bm0 = bsc[m] * bm0;
-      const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
-      double t;
-      t = 0.000001 + bp0; bp0 = d0 / (t * t);
-      t = 0.000001 + bm0; bm0 = d0 / (t * t);
-      t = 0.000001 + bp1; bp1 = d1 / (t * t);
-      t = 0.000001 + bm1; bm1 = d1 / (t * t);
-      double sump = bp0 + bp1, summ = bm0 + bm1;
-      bp0 = bp0 / sump; bp1 = bp1 / sump; bm0 = bm0 / summ; bm1 = bm1 / summ;
-      double vminus = bp0 * (-VP(m, i - 1) + 3 * VP(m, i)) + bp1 * (VP(m, i) + VP(m, i + 1));
-      double vplus = bm0 * (-VM(m, i + 2) + 3 * VM(m, i + 1)) + bm1 * (VM(m, i) + VM(m, i + 1));
-      vhat[m] = 0.5 * (vminus + vplus);
-    } else if (ord == 3) {
-      const double d0 = 13.0 / 12.0, d1 = 1.0 / 4.0;
-      const double c0 = 1.0 / 3.0, c1 = 5.0 / 6.0, c2 = -1.0 / 6.0, c3 = -7.0 / 6.0, c4 = 11.0 / 6.0;
-      const double dw0 = 1.0 / 10.0, dw1 = 6.0 / 10.0, dw2 = 3.0 / 10.0;
-      double a, bq;
-      a = VP(m, i) - 2.0 * VP(m, i + 1) + VP(m, i + 2); bq = 3.0 * VP(m, i) - 4.0 * VP(m, i + 1) + VP(m, i + 2);
-      double bp2 = d0 * a * a + d1 * bq * bq;
-      a = VP(m, i - 1) - 2.0 * VP(m, i) + VP(m, i + 1); bq = VP(m, i - 1) - VP(m, i + 1);
-      double bp1 = d0 * a * a + d1 * bq * bq;
-      a = VP(m, i) - 2.0 * VP(m, i - 1) + VP(m, i - 2); bq = 3.0 * VP(m, i) - 4.0 * VP(m, i - 1) + VP(m, i - 2);
-      double bp0 = d0 * a * a + d1 * bq * bq;
-      bp2 = bsc[m] * bp2; bp1 = bsc[m] * bp1; bp0 = bsc[m] * bp0;
-      a = VM(m, i + 1) - 2.0 * VM(m, i) + VM(m, i - 1); bq = 3.0 * VM(m, i + 1) - 4.0 * VM(m, i) + VM(m, i - 1);
-      double bm2 = d0 * a * a + d1 * bq * bq;
-      a = VM(m, i + 2) - 2.0 * VM(m, i + 1) + VM(m, i); bq = VM(m, i + 2) - VM(m, i);
-      double bm1 = d0 * a * a + d1 * bq * bq;
-      a = VM(m, i + 1) - 2.0 * VM(m, i + 2) + VM(m, i + 3); bq = 3.0 * VM(m, i + 1) - 4.0 * VM(m, i + 2) + VM(m, i + 3);
-      double bm0 = d0 * a * a + d1 * bq * bq;
-      bm2 = bsc[m] * bm2; bm1 = bsc[m] * bm1; bm0 = bsc[m] * bm0;
-      if (version == 0) {
-        double t;
-        t = 0.000001 + bp0; bp0 = dw0 / (t * t);
-        t = 0.000001 + bm0; bm0 = dw0 / (t * t);
-        t = 0.000001 + bp1; bp1 = dw1 / (t * t);
-        t = 0.000001 + bm1; bm1 = dw1 / (t * t);
-        t = 0.000001 + bp2; bp2 = dw2 / (t * t);
-        t = 0.000001 + bm2; bm2 = dw2 / (t * t);
-      } else {
-        const double eps40 = 1.0e-40;
-        double tau5p = fabs(bp0 - bp2) + eps40, tau5m = fabs(bm0 - bm2) + eps40;
-        bp0 = (bp0 + eps40) / (bp0 + tau5p); bm0 = (bm0 + eps40) / (bm0 + tau5m);
-        bp1 = (bp1 + eps40) / (bp1 + tau5p); bm1 = (bm1 + eps40) / (bm1 + tau5m);
-        bp2 = (bp2 + eps40) / (bp2 + tau5p); bm2 = (bm2 + eps40) / (bm2 + tau5m);
-        bp0 = dw0 / bp0; bm0 = dw0 / bm0;
-        bp1 = dw1 / bp1; bm1 = dw1 / bm1;
-        bp2 = dw2 / bp2; bm2 = dw2 / bm2;
-      }
-      double sump = bp0 + bp1 + bp2, summ = bm0 + bm1 + bm2;
-      bp0 = bp0 / sump; bp1 = bp1 / sump; bp2 = bp2 / sump;
-      bm0 = bm0 / summ; bm1 = bm1 / summ; bm2 = bm2 / summ;
-      double vminus = bp2 * (c0 * VP(m, i) + c1 * VP(m, i + 1) + c2 * VP(m, i + 2)) +
-                      bp1 * (c2 * VP(m, i - 1) + c1 * VP(m, i) + c0 * VP(m, i + 1)) +
-                      bp0 * (c0 * VP(m, i - 2) + c3 * VP(m, i - 1) + c4 * VP(m, i));
-      double vplus = bm2 * (c0 * VM(m, i + 1) + c1 * VM(m, i) + c2 * VM(m, i - 1)) +
-                     bm1 * (c2 * VM(m, i + 2) + c1 * VM(m, i + 1) + c0 * VM(m, i)) +
-                     bm0 * (c0 * VM(m, i + 3) + c3 * VM(m, i + 2) + c4 * VM(m, i + 1));
-      vhat[m] = vminus + vplus;
-    } else {  // ord == 4, WENO-7
-      const double d1 = 1.0 / 36.0, d2 = 13.0 / 12.0, d3 = 781.0 / 720.0;
-      const double dw[4] = {1.0 / 35.0, 12.0 / 35.0, 18.0 / 35.0, 4.0 / 35.0};
-      double bp[4], bm[4], a, bq, c;
-      a = -11 * VP(m, i) + 18 * VP(m, i + 1) - 9 * VP(m, i + 2) + 2 * VP(m, i + 3);
-      bq = 2 * VP(m, i) - 5 * VP(m, i + 1) + 4 * VP(m, i + 2) - VP(m, i + 3);
-      c = -VP(m, i) + 3 * VP(m, i + 1) - 3 * VP(m, i + 2) + VP(m, i + 3);
-      bp[3] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = -2 * VP(m, i - 1) - 3 * VP(m, i) + 6 * VP(m, i + 1) - VP(m, i + 2);
-      bq = VP(m, i - 1) - 2 * VP(m, i) + VP(m, i + 1);
-      c = -VP(m, i - 1) + 3 * VP(m, i) - 3 * VP(m, i + 1) + VP(m, i + 2);
-      bp[2] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = VP(m, i - 2) - 6 * VP(m, i - 1) + 3 * VP(m, i) + 2 * VP(m, i + 1);
-      bq = VP(m, i - 1) - 2 * VP(m, i) + VP(m, i + 1);
-      c = -VP(m, i - 2) + 3 * VP(m, i - 1) - 3 * VP(m, i) + VP(m, i + 1);
-      bp[1] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = -2 * VP(m, i - 3) + 9 * VP(m, i - 2) - 18 * VP(m, i - 1) + 11 * VP(m, i);
-      bq = -VP(m, i - 3) + 4 * VP(m, i - 2) - 5 * VP(m, i - 1) + 2 * VP(m, i);
-      c = -VP(m, i - 3) + 3 * VP(m, i - 2) - 3 * VP(m, i - 1) + VP(m, i);
-      bp[0] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = -11 * VM(m, i + 1) + 18 * VM(m, i) - 9 * VM(m, i - 1) + 2 * VM(m, i - 2);
-      bq = 2 * VM(m, i + 1) - 5 * VM(m, i) + 4 * VM(m, i - 1) - VM(m, i - 2);
-      c = -VM(m, i + 1) + 3 * VM(m, i) - 3 * VM(m, i - 1) + VM(m, i - 2);
-      bm[3] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = -2 * VM(m, i + 2) - 3 * VM(m, i + 1) + 6 * VM(m, i) - VM(m, i - 1);
-      bq = VM(m, i + 2) - 2 * VM(m, i + 1) + VM(m, i);
-      c = -VM(m, i + 2) + 3 * VM(m, i + 1) - 3 * VM(m, i) + VM(m, i - 1);
-      bm[2] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = VM(m, i + 3) - 6 * VM(m, i + 2) + 3 * VM(m, i + 1) + 2 * VM(m, i);
-      bq = VM(m, i + 2) - 2 * VM(m, i + 1) + VM(m, i);
-      c = -VM(m, i + 3) + 3 * VM(m, i + 2) - 3 * VM(m, i + 1) + VM(m, i);
-      bm[1] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      a = -2 * VM(m, i + 4) + 9 * VM(m, i + 3) - 18 * VM(m, i + 2) + 11 * VM(m, i + 1);
-      bq = -VM(m, i + 4) + 4 * VM(m, i + 3) - 5 * VM(m, i + 2) + 2 * VM(m, i + 1);
-      c = -VM(m, i + 4) + 3 * VM(m, i + 3) - 3 * VM(m, i + 2) + VM(m, i + 1);
-      bm[0] = d1 * a * a + d2 * bq * bq + d3 * c * c;
-      double sump = 0, summ = 0;
-      for (int l = 0; l < 4; ++l) {
-        bp[l] = bsc[m] * bp[l];
-        bm[l] = bsc[m] * bm[l];
-      }
-      for (int l = 0; l < 4; ++l) {
-        double t = 0.000001 + bp[l];
-        bp[l] = dw[l] / (t * t);
-        t = 0.000001 + bm[l];
-        bm[l] = dw[l] / (t * t);
-        sump = sump + bp[l];
-        summ = summ + bm[l];
-      }
-      for (int l = 0; l < 4; ++l) { bp[l] = bp[l] / sump; bm[l] = bm[l] / summ; }
-      double vminus = bp[3] * (6 * VP(m, i) + 26 * VP(m, i + 1) - 10 * VP(m, i + 2) + 2 * VP(m, i + 3)) +
-                      bp[2] * (-2 * VP(m, i - 1) + 14 * VP(m, i) + 14 * VP(m, i + 1) - 2 * VP(m, i + 2)) +
-                      bp[1] * (2 * VP(m, i - 2) - 10 * VP(m, i - 1) + 26 * VP(m, i) + 6 * VP(m, i + 1)) +
-                      bp[0] * (-6 * VP(m, i - 3) + 26 * VP(m, i - 2) - 46 * VP(m, i - 1) + 50 * VP(m, i));
-      double vplus = bm[3] * (6 * VM(m, i + 1) + 26 * VM(m, i) - 10 * VM(m, i - 1) + 2 * VM(m, i - 2)) +
-                     bm[2] * (-2 * VM(m, i + 2) + 14 * VM(m, i + 1) + 14 * VM(m, i) - 2 * VM(m, i - 1)) +
-                     bm[1] * (2 * VM(m, i + 3) - 10 * VM(m, i + 2) + 26 * VM(m, i + 1) + 6 * VM(m, i)) +
-                     bm[0] * (-6 * VM(m, i + 4) + 26 * VM(m, i + 3) - 46 * VM(m, i + 2) + 50 * VM(m, i + 1));
-      vhat[m] = (vminus + vplus) / 24.0;
+    double p8[8], m8[8];
+#pragma unroll
+    for (int k = -3; k <= 4; ++k) {
+      const int l = iweno + k;
+      p8[k + 3] = (l >= 1 && l <= 8) ? vp[m + NV * (l - 1)] : 0.0;
+      m8[k + 3] = (l >= 1 && l <= 8) ? vm[m + NV * (l - 1)] : 0.0;
     }
+    vhat[m] = wenorec1(p8, m8, bsc[m], ord, version);
   }
-#undef VP
-#undef VM
 }
 
-// WENO face flux (characteristic Roe/LLF or component-wise Rusanov)
-template <int NS, int DIR>
+// WENO face flux (characteristic Roe/LLF or component-wise Rusanov); RUS selects the
+// splitting at compile time, so that each kernel carries the registers of one path only
+template <int NS, int DIR, bool RUS>
 __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, double *fh) {
   using X = Idx<NS>;
   constexpr int NV = NS + 4;
   constexpr int JN = DIR == 1 ? X::J_U : DIR == 2 ? X::J_V : X::J_W;
   constexpr int IN = DIR == 1 ? X::I_U : DIR == 2 ? X::I_V : X::I_W;
   const int ws = A.weno_scheme, wsize = 2 * A.weno_scheme;
-  double gp[NV * 8], gm[NV * 8];
-  if (A.flux_splitting == 1) {
+  if constexpr (RUS) {
+    double gp[NV * 8], gm[NV * 8];
     double evmax = -1.0;
     for (int l = 1; l <= wsize; ++l) {
       int ll = f + l - ws;
@@ -412,40 +429,19 @@ __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, dou
   const double ta = DIR == 1 ? vv : uu, tb = DIR == 3 ? vv : ww;
   constexpr int IA = DIR == 1 ? X::I_V : X::I_U;
   constexpr int IB = DIR == 3 ? X::I_V : X::I_W;
-  // ---- left eigenvectors el(row, field) (eigenvectors_x/y/z)
-  double el[NV * NV];
+  // ---- characteristic projection of the split fluxes (the left eigenvectors of
+  // eigenvectors_x/y/z, :2703-2984, applied through their structure): with
+  // b1m_s = b1 + p_rho_s / c^2 and, per stencil cell, the sums S_b = sum_s b1m_s U_s,
+  // S = sum_s U_s, M = u U_u + v U_v + w U_w,
+  //   acoustic -/+:  (S_b +- un S / c) / 2 - (b2 M +- U_n / c) / 2 + b2 U_E / 2
+  //   species s:     U_s + Y_s (b2 (M - U_E) - S_b)
+  //   shear a / b:   U_a - ta S,  U_b - tb S
+  // for the conserved variables U and, the same way, the Euler fluxes F.  The species
+  // fields are projected one at a time right before their reconstruction.
+  double b1m[NS];
 #pragma unroll
-  for (int q = 0; q < NV * NV; ++q) el[q] = 0.0;
-#define EL(rr, cq) el[((rr) - 1) + NV * ((cq) - 1)]
-#pragma unroll
-  for (int s = 1; s <= NS; ++s) {
-    double b1i = b1 + prhoi[s - 1] * cci;
-    EL(s, 1) = 0.5 * (b1i + un * ci);
-    EL(s, NS + 2) = -ta;
-    EL(s, NS + 3) = -tb;
-    EL(s, NS + 4) = 0.5 * (b1i - un * ci);
-    EL(s, s + 1) = 1.0;
-    EL(X::I_U, 1 + s) = b2 * uu * yyroe[s - 1];
-    EL(X::I_V, 1 + s) = b2 * vv * yyroe[s - 1];
-    EL(X::I_W, 1 + s) = b2 * ww * yyroe[s - 1];
-    EL(X::I_E, 1 + s) = -b2 * yyroe[s - 1];
-#pragma unroll
-    for (int m = 1; m <= NS; ++m) {
-      double b1m = b1 + prhoi[m - 1] * cci;
-      EL(m, 1 + s) = EL(m, 1 + s) - b1m * yyroe[s - 1];
-    }
-  }
-  EL(X::I_U, 1) = -0.5 * (b2 * uu + (DIR == 1 ? ci : 0.0));
-  EL(X::I_V, 1) = -0.5 * (b2 * vv + (DIR == 2 ? ci : 0.0));
-  EL(X::I_W, 1) = -0.5 * (b2 * ww + (DIR == 3 ? ci : 0.0));
-  EL(X::I_E, 1) = 0.5 * b2;
-  EL(IA, NS + 2) = 1.0;
-  EL(IB, NS + 3) = 1.0;
-  EL(X::I_U, NS + 4) = -0.5 * (b2 * uu - (DIR == 1 ? ci : 0.0));
-  EL(X::I_V, NS + 4) = -0.5 * (b2 * vv - (DIR == 2 ? ci : 0.0));
-  EL(X::I_W, NS + 4) = -0.5 * (b2 * ww - (DIR == 3 ? ci : 0.0));
-  EL(X::I_E, NS + 4) = 0.5 * b2;
-  // ---- LLF eigenvalue bounds over the stencil
+  for (int s = 0; s < NS; ++s) b1m[s] = b1 + prhoi[s] * cci;
+  // LLF eigenvalue bounds over the stencil
   double ev1 = -1.0, ev2 = -1.0, ev4 = -1.0;
   for (int l = 1; l <= wsize; ++l) {
     int ll = f + l - ws;
@@ -454,38 +450,81 @@ __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, dou
     ev2 = fmax(fabs(unl), ev2);
     ev4 = fmax(fabs(unl + cl), ev4);
   }
-  // ---- characteristic projection of the split fluxes
-  for (int l = 1; l <= wsize; ++l) {
-    int ll = f + l - ws;
-    double rl = L(ll, X::J_R), ul = L(ll, X::J_U), vl = L(ll, X::J_V), wl = L(ll, X::J_W);
-    double hl = L(ll, X::J_H), pl = L(ll, X::J_P), unl = L(ll, JN);
-    double rhou = rl * unl;
-    double fi[NV], yl[NS];
+  double ap[8], am[8], zp[8], zm[8], sap[8], sam[8], sbp[8], sbm[8];   // acoustic -, +, shear a, b
+  double psu[8], psf[8], rlv[8], ruv[8];
 #pragma unroll
-    for (int s = 1; s <= NS; ++s) { yl[s - 1] = L(ll, s); fi[s - 1] = rhou * yl[s - 1]; }
-    fi[X::I_U - 1] = ul * rhou; fi[X::I_V - 1] = vl * rhou; fi[X::I_W - 1] = wl * rhou; fi[X::I_E - 1] = hl * rhou;
-    fi[IN - 1] = fi[IN - 1] + pl;
+  for (int d = -3; d <= 4; ++d) {
+    const int q = d + 3;
+    ap[q] = am[q] = zp[q] = zm[q] = sap[q] = sam[q] = sbp[q] = sbm[q] = 0.0;
+    psu[q] = psf[q] = rlv[q] = ruv[q] = 0.0;
+    if (d < 1 - ws || d > ws) continue;
+    const int ll = f + d;
+    const double rl = L(ll, X::J_R), ul = L(ll, X::J_U), vl = L(ll, X::J_V), wl = L(ll, X::J_W);
+    const double hl = L(ll, X::J_H), pl = L(ll, X::J_P), unl = L(ll, JN);
+    const double rhou = rl * unl;
+    double sbu = 0.0, su = 0.0, sbf = 0.0, sf = 0.0;
 #pragma unroll
-    for (int m = 1; m <= NV; ++m) {
-      double wc = 0.0, gc = 0.0;
-#pragma unroll
-      for (int s = 1; s <= NS; ++s) {
-        wc = wc + EL(s, m) * rl * yl[s - 1];
-        gc = gc + EL(s, m) * fi[s - 1];
-      }
-      wc = wc + EL(X::I_U, m) * rl * ul; gc = gc + EL(X::I_U, m) * fi[X::I_U - 1];
-      wc = wc + EL(X::I_V, m) * rl * vl; gc = gc + EL(X::I_V, m) * fi[X::I_V - 1];
-      wc = wc + EL(X::I_W, m) * rl * wl; gc = gc + EL(X::I_W, m) * fi[X::I_W - 1];
-      wc = wc + EL(X::I_E, m) * (rl * hl - pl); gc = gc + EL(X::I_E, m) * fi[X::I_E - 1];
-      double evm = (m == 1) ? ev1 : (m == NV ? ev4 : ev2);
-      double cq = 0.5 * (gc + evm * wc);
-      gp[(m - 1) + NV * (l - 1)] = cq;
-      gm[(m - 1) + NV * (l - 1)] = gc - cq;
+    for (int s = 0; s < NS; ++s) {
+      const double y = L(ll, s + 1);
+      const double us = rl * y, fs = rhou * y;
+      sbu = sbu + b1m[s] * us;
+      su = su + us;
+      sbf = sbf + b1m[s] * fs;
+      sf = sf + fs;
     }
+    const double Uu = rl * ul, Uv = rl * vl, Uw = rl * wl, Ue = rl * hl - pl;
+    double Fu = ul * rhou, Fv = vl * rhou, Fw = wl * rhou;
+    const double Fe = hl * rhou;
+    if (IN == X::I_U) Fu = Fu + pl;
+    else if (IN == X::I_V) Fv = Fv + pl;
+    else Fw = Fw + pl;
+    const double mu_ = uu * Uu + vv * Uv + ww * Uw, mf = uu * Fu + vv * Fv + ww * Fw;
+    const double Un = DIR == 1 ? Uu : DIR == 2 ? Uv : Uw, Fn = DIR == 1 ? Fu : DIR == 2 ? Fv : Fw;
+    const double Ua = IA == X::I_U ? Uu : Uv, Fa = IA == X::I_U ? Fu : Fv;
+    const double Ub = IB == X::I_V ? Uv : Uw, Fb = IB == X::I_V ? Fv : Fw;
+    const double unci = un * ci;
+    const double w1 = 0.5 * (sbu + unci * su) - 0.5 * (b2 * mu_ + ci * Un) + 0.5 * b2 * Ue;
+    const double g1 = 0.5 * (sbf + unci * sf) - 0.5 * (b2 * mf + ci * Fn) + 0.5 * b2 * Fe;
+    const double w4 = 0.5 * (sbu - unci * su) - 0.5 * (b2 * mu_ - ci * Un) + 0.5 * b2 * Ue;
+    const double g4 = 0.5 * (sbf - unci * sf) - 0.5 * (b2 * mf - ci * Fn) + 0.5 * b2 * Fe;
+    const double wa = Ua - ta * su, ga = Fa - ta * sf;
+    const double wb = Ub - tb * su, gb = Fb - tb * sf;
+    double cq;
+    cq = 0.5 * (g1 + ev1 * w1); ap[q] = cq; am[q] = g1 - cq;
+    cq = 0.5 * (g4 + ev4 * w4); zp[q] = cq; zm[q] = g4 - cq;
+    cq = 0.5 * (ga + ev2 * wa); sap[q] = cq; sam[q] = ga - cq;
+    cq = 0.5 * (gb + ev2 * wb); sbp[q] = cq; sbm[q] = gb - cq;
+    psu[q] = b2 * (mu_ - Ue) - sbu;
+    psf[q] = b2 * (mf - Fe) - sbf;
+    rlv[q] = rl;
+    ruv[q] = rhou;
   }
-#undef EL
+  // reconstruction field by field (wenorec's smoothness scales: 1/(rho0 u0)^2 for the
+  // acoustic and species fields, 1/(rho0^2 u0^4) for the shear fields)
+  const double u0_2 = A.u0 * A.u0, r2u2 = A.rho0 * A.rho0 * u0_2, r2u4 = r2u2 * u0_2;
+  const double bsc_a = 1.0 / r2u2, bsc_t = 1.0 / r2u4;
   double fc[NV];
-  wenorec<NS>(gp, gm, fc, ws, ord, A.weno_version, A.rho0, A.u0, false);
+  fc[0] = wenorec1(ap, am, bsc_a, ord, A.weno_version);
+#pragma unroll 1
+  for (int s = 0; s < NS; ++s) {
+    double p8[8], m8[8];
+    const double ys = yyroe[s];
+#pragma unroll
+    for (int d = -3; d <= 4; ++d) {
+      const int q = d + 3;
+      p8[q] = m8[q] = 0.0;
+      if (d < 1 - ws || d > ws) continue;
+      const double y = L(f + d, s + 1);
+      const double w = rlv[q] * y + ys * psu[q], g = ruv[q] * y + ys * psf[q];
+      const double cq = 0.5 * (g + ev2 * w);
+      p8[q] = cq;
+      m8[q] = g - cq;
+    }
+    fc[1 + s] = wenorec1(p8, m8, bsc_a, ord, A.weno_version);
+  }
+  fc[NS + 1] = wenorec1(sap, sam, bsc_t, ord, A.weno_version);
+  fc[NS + 2] = wenorec1(sbp, sbm, bsc_t, ord, A.weno_version);
+  fc[NS + 3] = wenorec1(zp, zm, bsc_a, ord, A.weno_version);
   // ---- back to conserved fluxes with the right eigenvectors er(field, row)
   // er(1,.) acoustic-, er(1+s,.) species, er(NS+2/NS+3,.) shear, er(NS+4,.) acoustic+
   double fsum_species = 0.0;  // unused accumulator kept explicit below
