@@ -314,6 +314,8 @@ def main():
                     help="reactor server on a Poisson stream of this many seconds (0: skip)")
     ap.add_argument("--c4", action="store_true",
                     help="also time the 512x256x128 14-species syngas mixing layer with ~10 %% WENO faces")
+    ap.add_argument("--c4-only", action="store_true",
+                    help="only the 512x256x128 mixing layer (for profiling the WENO pre-pass)")
     ap.add_argument("--reactors-only", action="store_true",
                     help="only the batched-chemistry part (for profiling the reactor kernel)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -330,6 +332,9 @@ def main():
 
     pkg = _pkg()
     import torch
+    if args.c4_only:
+        print(json.dumps({"c4_mixing_layer": c4_bench(pkg, steps=args.steps, warmup=args.warmup)}))
+        return
     if args.reactors_only:
         print(json.dumps({"batched_chemistry": reactor_bench(pkg, args.reactors, args.reactors_fp64)}))
         return

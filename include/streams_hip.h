@@ -145,7 +145,10 @@ int streams_bc_nr_lat(int dir, int start_or_end, int nr_type, int n_s, int nx, i
 /* euler_x (multideal_gpu.F90:948-1002): euler_x_fluxes_hybrid_kernel
  * (kernels_gpu.F90:88-425; flux_splitting=1: the Rusanov variant :427-730) on faces
  * istart_face..iend_face, then euler_x_update_cuf (:732-750) fl += (f_i - f_{i-1}) dcsidx
- * on cells eul_imin..eul_imax.  fhat is scratch: its content afterwards is unspecified. */
+ * on cells eul_imin..eul_imax.  fhat is scratch: its content afterwards is unspecified.
+ * With sensor_threshold < 1 the shocked-face worklist is stream-ordered scratch of 4 bytes
+ * per face (STREAMS_E_NOMEM if it cannot be allocated); the same holds for euler_y/z and
+ * rhs_sweep. */
 int streams_euler_x(int n_s, int nv, int nv_aux, int nx, int ny, int nz, int ng,
                     int istart_face, int iend_face, int lmax_base, int nkeep,
                     const double *rgas, const double *w_aux, const double *coeff_deriv1,

@@ -2,9 +2,12 @@
 //
 // Structure (per direction):
 //   1. shock pre-pass (only when the sensor threshold is below 1, the Ducros sensor's
-//      upper bound): one thread per face evaluates the sensor window and, on shocked
-//      faces, the characteristic (Roe) or component-wise (Rusanov) WENO flux, which it
-//      parks in fhat;
+//      upper bound), in two kernels: one thread per face evaluates the sensor window and
+//      appends the shocked faces to a worklist (one atomic per workgroup); then every
+//      lane of a persistent grid takes listed faces and parks their characteristic (Roe)
+//      or component-wise (Rusanov) WENO flux in fhat.  Shocked faces are a tenth of the
+//      faces in a typical shear layer but touch a third of the 64-face waves, so the
+//      list keeps the WENO lanes full instead of idling 60 % of each active wave;
 //   2. fused central pass: every face's KEEP split-form flux (or the parked WENO flux) and
 //      the flux-difference update of fl in the same kernel, so face fluxes never round-
 //      trip through HBM:
@@ -21,38 +24,119 @@
 namespace streams {
 
 // ---------------------------------------------------------------- shock pre-pass
-// one thread per face of every line; shocked faces get their WENO flux in fhat
-template <int NS, int DIR, bool RUS>
-__global__ void __launch_bounds__(128) weno_prepass_kernel(Box b, EulerArgs A, int t1n, long nfaces) {
-  using X = Idx<NS>;
-  constexpr int NV = NS + 4;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nf = A.fe - A.fs + 1;
-  if (t >= nfaces) return;
-  int f, t1, t2;
-  if (DIR == 1) {  // faces fastest (contiguous along x)
-    f = A.fs + (int)(t % nf);
+// Face t of the pre-pass enumeration -> (face f, transverse t1, t2): x faces fastest
+// (contiguous along x); y and z with i fastest, then the face, then the other index.
+template <int DIR>
+__device__ inline void prepass_face(long t, int fs, int nf, int t1n, int &f, int &t1, int &t2) {
+  if (DIR == 1) {
+    f = fs + (int)(t % nf);
     long r = t / nf;
     t1 = 1 + (int)(r % t1n);
     t2 = 1 + (int)(r / t1n);
-  } else {  // i fastest
+  } else {
     t1 = 1 + (int)(t % t1n);
     long r = t / t1n;
-    f = A.fs + (int)(r % nf);
+    f = fs + (int)(r % nf);
     t2 = 1 + (int)(r / nf);
   }
-  Line L{A.w_aux, 0, 0, b.pstride()};
-  line_base<DIR>(b, t1, t2, L.base, L.st);
-  if (!shocked<NS>(L, f, A.weno_scheme, A.sensor_threshold)) return;
-  int ord = A.weno_scheme + epo_at<DIR>(b, A.epo, t1, t2, f);
-  ord = ord < 1 ? 1 : ord;
-  double fh[NV];
-  weno_flux<NS, DIR, RUS>(L, f, A, ord, fh);
-  long fb, fst;
-  line_base<DIR>(b, t1, t2, fb, fst);
+}
+
+__global__ void zero_count_kernel(unsigned *c) {
+  if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+}
+
+// Sensor window of every face.  Shocked faces go to the worklist: with split set, faces
+// reconstructed at the full order WS_FAST fill it from the front (count[0]) and the
+// order-reduced ones from the back (count[1]); otherwise all from the front.  Order
+// within the list is arbitrary, which changes nothing: each face is computed on its own.
+constexpr int FLAG_BS = 256;
+constexpr int WS_FAST = 3;   // WENO5, specialised at compile time
+template <int NS, int DIR>
+__global__ void __launch_bounds__(FLAG_BS) weno_flag_kernel(Box b, EulerArgs A, int t1n, long nfaces, bool split,
+                                                            unsigned *__restrict__ list, unsigned *count) {
+  __shared__ unsigned wave_n[2][FLAG_BS / 64];
+  __shared__ unsigned block_base[2];
+  const long t = (long)blockIdx.x * FLAG_BS + threadIdx.x;
+  bool s = false, front = true;
+  if (t < nfaces) {
+    int f, t1, t2;
+    prepass_face<DIR>(t, A.fs, A.fe - A.fs + 1, t1n, f, t1, t2);
+    Line L{A.w_aux, 0, 0, b.pstride()};
+    line_base<DIR>(b, t1, t2, L.base, L.st);
+    s = shocked<NS>(L, f, A.weno_scheme, A.sensor_threshold);
+    if (s && split) front = epo_at<DIR>(b, A.epo, t1, t2, f) == 0;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long below_mask = (1ull << lane) - 1ull;
+  const unsigned long long mf = __ballot(s && front), mb = __ballot(s && !front);
+  if (lane == 0) {
+    wave_n[0][w] = (unsigned)__popcll(mf);
+    wave_n[1][w] = (unsigned)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned tot = 0;
+    for (int q = 0; q < FLAG_BS / 64; ++q) tot += wave_n[threadIdx.x][q];
+    block_base[threadIdx.x] = tot ? atomicAdd(count + threadIdx.x, tot) : 0u;
+  }
+  __syncthreads();
+  if (s) {
+    const int k = front ? 0 : 1;
+    unsigned off = block_base[k] + (unsigned)__popcll((front ? mf : mb) & below_mask);
+    for (int q = 0; q < w; ++q) off += wave_n[k][q];
+    if (front) list[off] = (unsigned)t;
+    else list[nfaces - 1 - (long)off] = (unsigned)t;
+  }
+}
+
+// WENO flux of the listed faces: entries q < count[k] of the front list (k = 0) or of the
+// back list (k = 1, stored from list[nfaces - 1] downwards).  The grid is a multiple of the
+// XCD count; the workgroups of one XCD share one contiguous eighth of the list, so faces
+// that share stencil cells (neighbours in the list) are read through the same L2.
+// WS > 0 fixes weno_scheme and the reconstruction order at compile time.
+constexpr int LIST_BS = 128;
+constexpr unsigned LIST_MAX_BLOCKS = 2048;
+template <int NS, int DIR, bool RUS, int WS>
+__global__ void __launch_bounds__(LIST_BS) weno_list_kernel(Box b, EulerArgs A, int t1n, long nfaces, int k,
+                                                            const unsigned *__restrict__ list, const unsigned *count) {
+  constexpr int NV = NS + 4;
+  const unsigned n = count[k];
+  const unsigned per_xcd = gridDim.x / NXCD;
+  const unsigned x = blockIdx.x % NXCD, r = blockIdx.x / NXCD;
+  const unsigned seg = (n + NXCD - 1) / NXCD;
+  const unsigned lo = x * seg;
+  const unsigned hi = lo + seg < n ? lo + seg : n;
+  const int nf = A.fe - A.fs + 1;
+  for (unsigned q = lo + r * LIST_BS + threadIdx.x; q < hi; q += per_xcd * LIST_BS) {
+    const long t = (long)(k == 0 ? list[q] : list[nfaces - 1 - (long)q]);
+    int f, t1, t2;
+    prepass_face<DIR>(t, A.fs, nf, t1n, f, t1, t2);
+    Line L{A.w_aux, 0, 0, b.pstride()};
+    line_base<DIR>(b, t1, t2, L.base, L.st);
+    double fh[NV];
+    if constexpr (WS > 0) {
+      weno_flux<NS, DIR, RUS, WS, WS>(L, f, A, WS, fh);
+    } else {
+      int ord = A.weno_scheme + epo_at<DIR>(b, A.epo, t1, t2, f);
+      ord = ord < 1 ? 1 : ord;
+      weno_flux<NS, DIR, RUS>(L, f, A, ord, fh);
+    }
 #pragma unroll
-  for (int m = 0; m < NV; ++m) A.fhat[fb + (long)f * fst + b.pstride() * m] = fh[m];
-  (void)X::J_R;
+    for (int m = 0; m < NV; ++m) A.fhat[L.base + (long)f * L.st + b.pstride() * m] = fh[m];
+  }
+}
+
+template <int NS, int DIR, bool RUS>
+inline void list_launch(const Box &b, const EulerArgs &A, int t1n, long nfaces, bool split, const unsigned *list,
+                        const unsigned *count, hipStream_t st) {
+  unsigned nb = blocks_for(nfaces, LIST_BS);
+  nb = xcd_grid(nb < LIST_MAX_BLOCKS ? nb : LIST_MAX_BLOCKS);
+  if (split) {
+    weno_list_kernel<NS, DIR, RUS, WS_FAST><<<nb, LIST_BS, 0, st>>>(b, A, t1n, nfaces, 0, list, count);
+    weno_list_kernel<NS, DIR, RUS, 0><<<NXCD * 8, LIST_BS, 0, st>>>(b, A, t1n, nfaces, 1, list, count);
+  } else {
+    weno_list_kernel<NS, DIR, RUS, 0><<<nb, LIST_BS, 0, st>>>(b, A, t1n, nfaces, 0, list, count);
+  }
 }
 
 // ---------------------------------------------------------------- fused x sweep
@@ -125,11 +209,23 @@ __global__ void __launch_bounds__(128) euler_yz_fused_kernel(Box b, EulerArgs A,
 }
 
 template <int NS, int DIR>
-inline void prepass_launch(const Box &b, const EulerArgs &A, int t1n, long nfaces, hipStream_t st) {
+inline int prepass_launch(const Box &b, const EulerArgs &A, int t1n, long nfaces, hipStream_t st) {
+  if (nfaces > (long)0xFFFFFF00u) return STREAMS_E_DIMS;
+  // stream-ordered worklist: calls on different streams each get their own
+  unsigned *buf = nullptr;
+  if (hipMallocAsync((void **)&buf, ((size_t)nfaces + 64) * sizeof(unsigned), st) != hipSuccess)
+    return STREAMS_E_NOMEM;
+  unsigned *count = buf, *list = buf + 64;
+  const bool split = A.weno_scheme == WS_FAST;
+  zero_count_kernel<<<1, 64, 0, st>>>(count);
+  weno_flag_kernel<NS, DIR><<<blocks_for(nfaces, FLAG_BS), FLAG_BS, 0, st>>>(b, A, t1n, nfaces, split, list, count);
   if (A.flux_splitting == 1)
-    weno_prepass_kernel<NS, DIR, true><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces);
+    list_launch<NS, DIR, true>(b, A, t1n, nfaces, split, list, count, st);
   else
-    weno_prepass_kernel<NS, DIR, false><<<blocks_for(nfaces, 128), 128, 0, st>>>(b, A, t1n, nfaces);
+    list_launch<NS, DIR, false>(b, A, t1n, nfaces, split, list, count, st);
+  const int rc = hip_status();
+  if (hipFreeAsync(buf, st) != hipSuccess && rc == 0) return hip_status();
+  return rc;
 }
 
 int weno_prepass_launch(int dir, int ns, const Box &b, const EulerArgs &A, hipStream_t st) {
@@ -138,10 +234,11 @@ int weno_prepass_launch(int dir, int ns, const Box &b, const EulerArgs &A, hipSt
   const int t2n = dir == 3 ? b.ny : b.nz;
   const long nfaces = (long)nf * t1n * t2n;
   if (nfaces <= 0) return 0;
-  if (dir == 1) { STREAMS_DISPATCH_NS(ns, (prepass_launch<NS, 1>(b, A, t1n, nfaces, st))); }
-  else if (dir == 2) { STREAMS_DISPATCH_NS(ns, (prepass_launch<NS, 2>(b, A, t1n, nfaces, st))); }
-  else { STREAMS_DISPATCH_NS(ns, (prepass_launch<NS, 3>(b, A, t1n, nfaces, st))); }
-  return hip_status();
+  int rc = 0;
+  if (dir == 1) { STREAMS_DISPATCH_NS(ns, (rc = prepass_launch<NS, 1>(b, A, t1n, nfaces, st))); }
+  else if (dir == 2) { STREAMS_DISPATCH_NS(ns, (rc = prepass_launch<NS, 2>(b, A, t1n, nfaces, st))); }
+  else { STREAMS_DISPATCH_NS(ns, (rc = prepass_launch<NS, 3>(b, A, t1n, nfaces, st))); }
+  return rc;
 }
 
 template <int NS, int DIR>
@@ -153,8 +250,7 @@ int euler_launch(const Box &b, EulerArgs A, hipStream_t st) {
   const int t2n = DIR == 3 ? b.ny : b.nz;
   const long nfaces = (long)nf * t1n * t2n;
   if (check) {
-    prepass_launch<NS, DIR>(b, A, t1n, nfaces, st);
-    int rc = hip_status();
+    int rc = prepass_launch<NS, DIR>(b, A, t1n, nfaces, st);
     if (rc) return rc;
   }
   if (DIR == 1) {

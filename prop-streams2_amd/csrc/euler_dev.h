@@ -325,13 +325,16 @@ __device__ void wenorec(const double *vp, const double *vm, double *vhat, int iw
 
 // WENO face flux (characteristic Roe/LLF or component-wise Rusanov); RUS selects the
 // splitting at compile time, so that each kernel carries the registers of one path only
-template <int NS, int DIR, bool RUS>
-__device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, double *fh) {
+// WS > 0: weno_scheme known at compile time (the stencil slots outside it fold to zero);
+// ORD > 0: the reconstruction order known as well (no order reduction at this face).
+template <int NS, int DIR, bool RUS, int WS = 0, int ORD = 0>
+__device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord_rt, double *fh) {
   using X = Idx<NS>;
   constexpr int NV = NS + 4;
   constexpr int JN = DIR == 1 ? X::J_U : DIR == 2 ? X::J_V : X::J_W;
   constexpr int IN = DIR == 1 ? X::I_U : DIR == 2 ? X::I_V : X::I_W;
-  const int ws = A.weno_scheme, wsize = 2 * A.weno_scheme;
+  const int ws = WS > 0 ? WS : A.weno_scheme, wsize = 2 * ws;
+  const int ord = ORD > 0 ? ORD : ord_rt;
   if constexpr (RUS) {
     double gp[NV * 8], gm[NV * 8];
     double evmax = -1.0;
@@ -438,9 +441,6 @@ __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, dou
   //   shear a / b:   U_a - ta S,  U_b - tb S
   // for the conserved variables U and, the same way, the Euler fluxes F.  The species
   // fields are projected one at a time right before their reconstruction.
-  double b1m[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) b1m[s] = b1 + prhoi[s] * cci;
   // LLF eigenvalue bounds over the stencil
   double ev1 = -1.0, ev2 = -1.0, ev4 = -1.0;
   for (int l = 1; l <= wsize; ++l) {
@@ -467,9 +467,10 @@ __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, dou
     for (int s = 0; s < NS; ++s) {
       const double y = L(ll, s + 1);
       const double us = rl * y, fs = rhou * y;
-      sbu = sbu + b1m[s] * us;
+      const double b1m = b1 + prhoi[s] * cci;
+      sbu = sbu + b1m * us;
       su = su + us;
-      sbf = sbf + b1m[s] * fs;
+      sbf = sbf + b1m * fs;
       sf = sf + fs;
     }
     const double Uu = rl * ul, Uv = rl * vl, Uw = rl * wl, Ue = rl * hl - pl;
@@ -505,6 +506,9 @@ __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, dou
   const double bsc_a = 1.0 / r2u2, bsc_t = 1.0 / r2u4;
   double fc[NV];
   fc[0] = wenorec1(ap, am, bsc_a, ord, A.weno_version);
+  fc[NS + 1] = wenorec1(sap, sam, bsc_t, ord, A.weno_version);
+  fc[NS + 2] = wenorec1(sbp, sbm, bsc_t, ord, A.weno_version);
+  fc[NS + 3] = wenorec1(zp, zm, bsc_a, ord, A.weno_version);
 #pragma unroll 1
   for (int s = 0; s < NS; ++s) {
     double p8[8], m8[8];
@@ -522,13 +526,8 @@ __device__ void weno_flux(const Line &L, int f, const EulerArgs &A, int ord, dou
     }
     fc[1 + s] = wenorec1(p8, m8, bsc_a, ord, A.weno_version);
   }
-  fc[NS + 1] = wenorec1(sap, sam, bsc_t, ord, A.weno_version);
-  fc[NS + 2] = wenorec1(sbp, sbm, bsc_t, ord, A.weno_version);
-  fc[NS + 3] = wenorec1(zp, zm, bsc_a, ord, A.weno_version);
   // ---- back to conserved fluxes with the right eigenvectors er(field, row)
   // er(1,.) acoustic-, er(1+s,.) species, er(NS+2/NS+3,.) shear, er(NS+4,.) acoustic+
-  double fsum_species = 0.0;  // unused accumulator kept explicit below
-  (void)fsum_species;
 #pragma unroll
   for (int m = 1; m <= NV; ++m) {
     double s = 0.0;
