@@ -36,8 +36,11 @@ __device__ inline bool cell_of(const Box &b, long t, int &i, int &j, int &k, lon
   return true;
 }
 
+// 6 waves per SIMD for up to 9 species (80 VGPRs, 12 B/lane of spills: faster than the
+// compiler's own 5); beyond 9 species those 80 VGPRs spill 144-224 B/lane, and 4 waves
+// (about 100 VGPRs, no spills) is the fastest measured.
 template <int NS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) finish_species_kernel(Box b, double *__restrict__ w,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS <= 9 ? 6 : 4))) finish_species_kernel(Box b, double *__restrict__ w,
                                                              const double *__restrict__ wa, double *__restrict__ fl,
                                                              double *__restrict__ fln, FinishArgs F) {
   using X = Idx<NS>;
