@@ -263,38 +263,76 @@ def serving_bench(pkg, duration: float, rate: float = 200.0, budget: int = 200):
     return st
 
 
-def c4_bench(pkg, shape=(512, 256, 128), steps: int = 2, warmup: int = 1):
-    """The 14-species Keromnes-2013 syngas mixing layer on one rank's 512 x 256 x 128 box:
-    RK3 steps of the fused path with the sensor threshold set so that about 10 % of the
-    x faces take the WENO flux (WENO5, Roe splitting), cell-updates/s."""
+def c4_bench(pkg, shape=(512, 256, 128), steps: int = 2, warmup: int = 1, explicit: bool = True):
+    """The 14-species Keromnes-2013 syngas mixing layer on one rank's 512 x 256 x 128 box, with
+    the sensor threshold set so that about 10 % of the x faces take the WENO flux (WENO5, Roe
+    splitting): Strang steps (chemistry dt/2, RK3 transport, chemistry dt/2) with the
+    interior cells' implicit chemistry run as one mixed-precision reactor batch, in
+    cell-updates/s; with explicit, also RK3 steps with the explicit chemistry source."""
     import torch
     tb = pkg.cases.keromnes_mechanism().tables()
     nx, ny, nz = shape
     print(f"[bench] building the {nx}x{ny}x{nz} syngas mixing layer", file=sys.stderr, flush=True)
     w, tg, info = pkg.cases.mixing_layer(tb, nx, ny, nz, seed=2024)
     be = pkg.solver.HipBackend("cuda:0")
-    num = pkg.solver.Numerics(weno_scheme=3)
+    num = pkg.solver.Numerics(weno_scheme=3, enable_chemistry=2, operator_splitting=2, chem_batch_form=1)
     s = pkg.solver.Solver(be, tb, nx, ny, nz, 3, info["length"], numerics=num, rho0=info["rho0"], u0=info["u0"])
     s.set_state(w, tg)
+    w0 = s.w.clone()
     del w, tg
     s.compute_aux()
     duc = be.to_numpy(s.w_aux[tb.n_species + 8])[3:-3, 3:-3, 3:-3]
     num.sensor_threshold = pkg.cases.weno_threshold_for_fraction(duc, num.weno_scheme, 0.10)
     del duc
-    s.compute_dt(0.5)
-    for _ in range(warmup):
-        s.rk_sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s.rk_sync()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    return {"workload": f"{nx}x{ny}x{nz} H2/CO-air temporal mixing layers, Keromnes-2013 (14 species, "
-                        f"37 reactions), M_c 0.8, WENO5 on ~10 % of the faces, KEEP elsewhere",
-            "cells": nx * ny * nz, "steps": steps, "ms_per_step": 1e3 * el / steps,
-            "cell_updates_per_s": 3 * nx * ny * nz * steps / el, "sensor_threshold": num.sensor_threshold,
-            "dt_s": s.dt}
+    dt = s.compute_dt(0.5)
+
+    def timed(step):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    print("[bench] Strang steps, batched mixed-precision chemistry", file=sys.stderr, flush=True)
+    el = timed(s.advance)
+    out = {"workload": f"{nx}x{ny}x{nz} H2/CO-air temporal mixing layers, Keromnes-2013 (14 species, "
+                       f"37 reactions), M_c 0.8, WENO5 on ~10 % of the faces, KEEP elsewhere; step = Strang "
+                       f"step: implicit chemistry of every cell over dt/2 as one mixed-precision reactor batch, "
+                       f"Wray RK3 transport, chemistry over dt/2",
+           "cells": nx * ny * nz, "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "cell_updates_per_s": 3 * nx * ny * nz * steps / el, "sensor_threshold": num.sensor_threshold,
+           "dt_s": dt}
+    # one chemistry half-step of the last state in each batch form: the precision speed-up on
+    # field cells (form 1 mixed, form 2 FP64 species block, form 0 FP64 reference arithmetic)
+    jt = tb.n_species + 4
+    w_s, t_s = s.w.clone(), s.w_aux[jt].clone()
+    half = {}
+    for form in (1, 2, 0):
+        s.w.copy_(w_s)
+        s.w_aux[jt].copy_(t_s)
+        num.chem_batch_form = form
+        print(f"[bench] one chemistry half-step, batch form {form}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.chemistry_batch(0.5 * dt)
+        torch.cuda.synchronize()
+        half[form] = time.perf_counter() - t0
+    num.chem_batch_form = 1
+    out["chemistry_half_step_ms"] = {"mixed": 1e3 * half[1], "fp64_species_block": 1e3 * half[2],
+                                     "fp64_reference": 1e3 * half[0]}
+    if explicit:
+        print("[bench] RK3 steps, explicit chemistry source", file=sys.stderr, flush=True)
+        s.w.copy_(w0)
+        num.enable_chemistry = 1
+        s.compute_aux()
+        s.dt = dt
+        el = timed(s.rk_sync)
+        out["explicit_chemistry"] = {"ms_per_step": 1e3 * el / steps,
+                                     "cell_updates_per_s": 3 * nx * ny * nz * steps / el}
+    return out
 
 
 def main():

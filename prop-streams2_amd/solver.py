@@ -62,6 +62,9 @@ class HipBackend:
     def copy_(self, dst, src):
         dst.copy_(src)
 
+    def fill_(self, dst, value: float):
+        dst.fill_(value)
+
     def sync(self):
         self.torch.cuda.synchronize(self.device)
 
@@ -90,6 +93,11 @@ class Numerics:
     ros_maxtry: int = 40
     rosenbrock_version: int = 1      # 1: ROS4 (rosenbrock), 2: Rosenbrock-Krylov
     operator_splitting: int = 2      # with enable_chemistry == 2: 1 simpler (Krylov only), 2 Strang
+    # with enable_chemistry == 2 and Strang splitting: -1 integrates each cell with the
+    # reference's field kernel (rosenbrock_version); 0, 1, 2 integrate the interior cells as
+    # one batch of constant-volume reactors (streams_reactor_batch form `mixed`: 0 FP64 in
+    # the reference's arithmetic, 1 mixed precision, 2 FP64 species block)
+    chem_batch_form: int = -1
     # [lespar] (multideal.F90:615-646): enable_les with les_model 1 (WALE) or 2 (WALE +
     # PaSR closure of the implicit chemistry); the constants have no defaults there
     enable_les: int = 0
@@ -373,6 +381,33 @@ class Solver:
                      0, n.enable_pasr, n.les_c_yoshi, n.les_c_mix, n.les_c_eps, self.dxi[0], self.dxi[1], self.dxi[2],
                      self.be.stream)
 
+    def chemistry_batch(self, dt_chem: float):
+        """The implicit chemistry of one Strang half-step with every interior cell as one
+        reactor of a batch (streams_reactor_batch, form Numerics.chem_batch_form): the
+        conserved variables and the temperature guess are copied out of the padded field
+        (streams_field_interior), integrated over dt_chem from the first trial step dt_chem
+        as the field kernel rosenbrock_kernel does (kernels_gpu.F90:8928-9193), and copied
+        back with the last temperature of each reactor."""
+        n, be = self.num, self.be
+        if n.enable_pasr:
+            raise ValueError("the reactor batch has no PaSR closure: use chem_batch_form = -1")
+        if be.name != "hip":
+            raise ValueError("chem_batch_form >= 0 runs the HIP reactor batch; this backend has none")
+        nb = self.nx * self.ny * self.nz
+        if getattr(self, "_chem_batch", None) is None:
+            from .reactors import ReactorBatchGPU
+            rb = ReactorBatchGPU(be, self.tb, tol=n.ros_eps, maxsteps=n.ros_maxsteps, maxtry=n.ros_maxtry)
+            self._chem_batch = (rb, be.zeros((self.nv, nb)), be.zeros((nb,)), be.zeros((nb,)), be.zeros((nb,)))
+        rb, wb, tgb, dtb, teb = self._chem_batch
+        jt = self.ns + 4                                   # J_T (0-based)
+        be.call("field_interior", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, wb, 1, be.stream)
+        be.call("field_interior", self.nx, self.ny, self.nz, self.ng, 1, self.w_aux[jt], tgb, 1, be.stream)
+        be.fill_(dtb, dt_chem)
+        be.fill_(teb, dt_chem)
+        rb.integrate(wb, tgb, dtb, teb, mixed=n.chem_batch_form)
+        be.call("field_interior", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, wb, 0, be.stream)
+        be.call("field_interior", self.nx, self.ny, self.nz, self.ng, 1, self.w_aux[jt], tgb, 0, be.stream)
+
     def rosenbrock_krylov(self, dt_chem: float, fl_sav=None):
         """rosenbrock_krylov (multideal_gpu.F90:769-838); fl_sav given: simpler splitting,
         the transport forcing subtracted from every chemistry right-hand side."""
@@ -535,6 +570,9 @@ class Solver:
         if n.enable_chemistry == 2 and n.operator_splitting == 1:
             if n.rosenbrock_version != 2:
                 raise ValueError("simpler splitting needs the Rosenbrock-Krylov integrator (rosenbrock_version 2)")
+            if n.chem_batch_form >= 0:
+                raise ValueError("the reactor batch takes no transport forcing: simpler splitting needs "
+                                 "chem_batch_form = -1")
             dt = self.dt
             self.dt = 0.5 * dt
             self.sav_flx()
@@ -544,7 +582,10 @@ class Solver:
             self.dt = dt
         elif n.enable_chemistry == 2:
             dt_chem = 0.5 * self.dt
-            chem = self.rosenbrock if n.rosenbrock_version == 1 else self.rosenbrock_krylov
+            if n.chem_batch_form >= 0:
+                chem = self.chemistry_batch
+            else:
+                chem = self.rosenbrock if n.rosenbrock_version == 1 else self.rosenbrock_krylov
             chem(dt_chem)
             self.rk_sync(fused=fused)
             self.compute_aux(central=1, ghost=0)

@@ -138,3 +138,30 @@ def test_field_info_values_round_trip(tmp_path):
     back = rs.read_field_info(tmp_path / "f.dat")
     assert back == info
     assert "time_from_last_rst = 3.3333333333333331E-001" in (tmp_path / "f.dat").read_text()
+
+
+def test_batched_chemistry_options_are_checked(pkg, be):
+    """chem_batch_form >= 0 runs the HIP reactor batch: a backend without one, the simpler
+    splitting (which needs the transport forcing the batch does not take) and the PaSR
+    closure are refused before any work is done."""
+    tb = pkg.cases.h2_air_mechanism().tables()
+    w, tg, info = pkg.cases.turbulence_box(tb, 6, seed=3, ng=3)
+
+    def solver(**kw):
+        num = pkg.solver.Numerics(enable_chemistry=2, chem_batch_form=1, **kw)
+        s = pkg.solver.Solver(be, tb, 6, 6, 6, 3, [info["length"]] * 3, numerics=num, rho0=info["rho0"],
+                              u0=info["u_rms"])
+        s.set_state(w, tg)
+        s.compute_aux()
+        s.dt = 1e-9
+        return s
+
+    w0 = solver().w.copy()
+    with pytest.raises(ValueError, match="HIP reactor batch"):
+        s = solver()
+        s.advance()
+    assert np.array_equal(s.w, w0)
+    with pytest.raises(ValueError, match="simpler splitting"):
+        solver(operator_splitting=1, rosenbrock_version=2).advance()
+    with pytest.raises(ValueError, match="PaSR"):
+        solver(enable_les=1, les_model=2).chemistry_batch(1e-9)
