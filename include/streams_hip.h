@@ -386,8 +386,10 @@ int streams_find_boom(int nx, int ny, int nz, int ng, int nv, const double *fln,
 
 /* Convective (euler_{x,y,z}_fluxes_hybrid + update) and, with visc=1, viscous
  * (visflx_{x,y,z}_cuf) fluxes of direction dir (1..3) on faces 0..n with one update of
- * fl for cells 1..n; init=1 also performs init_flux_cuf (fln = -rhodt*fl_old).  metric is
- * dcsidx/detady/dzitdz, coord is x/y/z of that direction. */
+ * fl for cells 1..n; init=1 also performs init_flux_cuf (fln = -rhodt*fl_old); init=2
+ * sets fl to this direction's part without reading it or touching fln, for a substep whose
+ * fln streams_rhs_finish_next already set.  metric is dcsidx/detady/dzitdz, coord is x/y/z
+ * of that direction. */
 int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, int nz, int ng, int lmax_base,
                       int nkeep, const double *rgas, const double *w_aux, const double *coeff_deriv1,
                       const double *metric, const double *coord, double *fhat,
@@ -421,6 +423,20 @@ int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int 
                        const double *mw, const double *kin_pack, const double *kc_tab,
                        int num_t_tab, double t_min_tab, double dt_tab, double r_univ,
                        double gamdt, const int *fluid_mask, void *stream);
+
+/* streams_rhs_finish followed by the next substep's init_flux_cuf (:28-45), for the
+ * substeps of one rk_sync but its last: fln receives -rhodt_next * fl (fl the substep's
+ * final right-hand side) in place of the update increment, and fl is not stored.  The next
+ * substep's first streams_rhs_sweep then runs with init=2.  w is updated as by
+ * streams_rhs_finish, and the pair gives bit-identical results to streams_rhs_finish and a
+ * sweep with init=1. */
+int streams_rhs_finish_next(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
+                            double *w, const double *w_aux, double *fl, double *fln,
+                            const double *coeff_deriv1, const double *dcsidx, const double *detady,
+                            const double *dzitdz, int reduced, int chem, int nreactions,
+                            const double *mw, const double *kin_pack, const double *kc_tab,
+                            int num_t_tab, double t_min_tab, double dt_tab, double r_univ,
+                            double gamdt, double rhodt_next, const int *fluid_mask, void *stream);
 
 /* ---- batched chemistry ---------------------------------------------------------------- */
 /* nbatch independent reactors, each integrated like one cell of rosenbrock_kernel

@@ -62,6 +62,7 @@ _ARGS = {
     "find_boom": "iiiii" "p" "d" "p" "ii" "PP" "s",
     "rhs_sweep": "iiiiiiiii" "ppppp" "p" "iiii" "d" "pp" "i" "p" "ii" "p" "ddd" "ii" "d" "i" "d" "pp" "s",
     "rhs_finish": "iiiiiii" "pppppppp" "iii" "ppp" "i" "dddd" "p" "s",
+    "rhs_finish_next": "iiiiiii" "pppppppp" "iii" "ppp" "i" "ddddd" "p" "s",
     "reactor_batch": "ii" "pppp" "ppp" "p" "i" "pp" "i" "dddd" "p" "i" "p" "ii" "ii" "d" "pp" "pp" "p" "ii" "s",
 }
 

@@ -18,6 +18,10 @@ struct FinishArgs {
   KinPacked K;
   int lmax, reduced, chem;
   double gamdt;
+  // fold = 1: the next substep's init_flux is done here (fln = -rhodt_next * fl, written in
+  // place of fl, which then needs no store: the next x sweep overwrites it with init = 2)
+  int fold;
+  double rhodt_next;
 };
 
 // One interior cell per thread, i fastest, so that every access of a wave is a coalesced
@@ -66,9 +70,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS <= 
   const bool fluid = F.mask == nullptr || F.mask[pt] == 0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    fl[t + fs_ * s] = acc[s];
     const double fn = fln[t + fs_ * s] - gamdt * acc[s];
-    fln[t + fs_ * s] = fn;
+    if (F.fold) {
+      fln[t + fs_ * s] = -F.rhodt_next * acc[s];
+    } else {
+      fl[t + fs_ * s] = acc[s];
+      fln[t + fs_ * s] = fn;
+    }
     if (fluid) w[pt + ps * s] = w[pt + ps * s] + fn;
   }
 }
@@ -152,9 +160,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) f
   for (int m = 0; m < 4; ++m) {
     const long q = t + fs_ * (NS + m);
     const double acc = F.reduced ? fl[q] - red[m] : fl[q];
-    fl[q] = acc;
     const double fn = fln[q] - gamdt * acc;
-    fln[q] = fn;
+    if (F.fold) {
+      fln[q] = -F.rhodt_next * acc;
+    } else {
+      fl[q] = acc;
+      fln[q] = fn;
+    }
     if (fluid) w[pt + ps * (NS + m)] = w[pt + ps * (NS + m)] + fn;
   }
 }
@@ -214,13 +226,13 @@ extern "C" int streams_kinetics_pack(int n_s, int nreactions, const double *arr_
 }
 
 // Tail of one RK substep: reduced viscous terms (reduced=1), explicit chemistry (chem=1),
-// update_flux and update_field, in one pass.
-extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order, double *w,
-                                  const double *w_aux, double *fl, double *fln, const double *coeff_deriv1,
-                                  const double *dcsidx, const double *detady, const double *dzitdz, int reduced,
-                                  int chem, int nreactions, const double *mw, const double *kin_pack,
-                                  const double *kc_tab, int num_t_tab, double t_min_tab, double dt_tab,
-                                  double r_univ, double gamdt, const int *fluid_mask, void *stream) {
+// update_flux and update_field, in one pass; fold=1 also does the next substep's init_flux.
+static int rhs_finish_impl(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order, double *w,
+                           const double *w_aux, double *fl, double *fln, const double *coeff_deriv1,
+                           const double *dcsidx, const double *detady, const double *dzitdz, int reduced, int chem,
+                           int nreactions, const double *mw, const double *kin_pack, const double *kc_tab,
+                           int num_t_tab, double t_min_tab, double dt_tab, double r_univ, double gamdt, int fold,
+                           double rhodt_next, const int *fluid_mask, void *stream) {
   if (nv_aux < 2 * n_s + 13) return STREAMS_E_DIMS;
   const int lmax = visc_order / 2;
   if (reduced && (lmax < 1 || lmax > 4 || lmax > ng)) return STREAMS_E_OPTION;
@@ -230,7 +242,7 @@ extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int n
   if (n == 0) return 0;
   FinishArgs F{coeff_deriv1, dcsidx, detady, dzitdz, fluid_mask, mw,
                KinPacked{kin_pack, kc_tab, nreactions, num_t_tab, t_min_tab, dt_tab, r_univ},
-               lmax, reduced, chem && nreactions > 0, gamdt};
+               lmax, reduced, chem && nreactions > 0, gamdt, fold, rhodt_next};
   const StencilRows SR = stencil_rows(b);
   if (SR.tiles(nz) > 0xffffff00u) return STREAMS_E_DIMS;
   const unsigned gm = xcd_grid(SR.tiles(nz));
@@ -238,4 +250,27 @@ extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int n
                             (lmax == 3 ? finish_mom_kernel<NS, 3><<<gm, 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F, SR)
                                        : finish_mom_kernel<NS, 4><<<gm, 256, 0, as_stream(stream)>>>(b, w, w_aux, fl, fln, F, SR))));
   return hip_status();
+}
+
+extern "C" int streams_rhs_finish(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order, double *w,
+                                  const double *w_aux, double *fl, double *fln, const double *coeff_deriv1,
+                                  const double *dcsidx, const double *detady, const double *dzitdz, int reduced,
+                                  int chem, int nreactions, const double *mw, const double *kin_pack,
+                                  const double *kc_tab, int num_t_tab, double t_min_tab, double dt_tab,
+                                  double r_univ, double gamdt, const int *fluid_mask, void *stream) {
+  return rhs_finish_impl(n_s, nx, ny, nz, ng, nv_aux, visc_order, w, w_aux, fl, fln, coeff_deriv1, dcsidx, detady,
+                         dzitdz, reduced, chem, nreactions, mw, kin_pack, kc_tab, num_t_tab, t_min_tab, dt_tab,
+                         r_univ, gamdt, 0, 0.0, fluid_mask, stream);
+}
+
+extern "C" int streams_rhs_finish_next(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
+                                       double *w, const double *w_aux, double *fl, double *fln,
+                                       const double *coeff_deriv1, const double *dcsidx, const double *detady,
+                                       const double *dzitdz, int reduced, int chem, int nreactions,
+                                       const double *mw, const double *kin_pack, const double *kc_tab,
+                                       int num_t_tab, double t_min_tab, double dt_tab, double r_univ, double gamdt,
+                                       double rhodt_next, const int *fluid_mask, void *stream) {
+  return rhs_finish_impl(n_s, nx, ny, nz, ng, nv_aux, visc_order, w, w_aux, fl, fln, coeff_deriv1, dcsidx, detady,
+                         dzitdz, reduced, chem, nreactions, mw, kin_pack, kc_tab, num_t_tab, t_min_tab, dt_tab,
+                         r_univ, gamdt, 1, rhodt_next, fluid_mask, stream);
 }

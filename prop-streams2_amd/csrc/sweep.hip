@@ -39,7 +39,7 @@ struct SweepArgs {
   double *fln;
   unsigned ncb, nseg, ntiles;   // column tiles, segments along the sweep, their product with the third extent
   double rhodt;
-  int init;          // fold init_flux: fln = -rhodt*fl_old, fl = contribution
+  int init;          // 1: fold init_flux (fln = -rhodt*fl_old, fl = contribution); 2: fl = contribution
   int visc;          // add the viscous flux
   int check_shock;   // sensor threshold below 1: consult the pre-pass results
 };
@@ -397,13 +397,16 @@ __global__ void __launch_bounds__(256) sweep_kernel(Box b, const double *__restr
   else if (DIR == 2) fcell = b.f(col, cell, t2);
   else fcell = b.f(col, t2, cell);
   const long fs_ = b.fstride();
-  if (A.init) {
+  if (A.init == 1) {
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
       const double old = fl[fcell + fs_ * m];
       fln[fcell + fs_ * m] = -A.rhodt * old;
       fl[fcell + fs_ * m] = acc[m];
     }
+  } else if (A.init == 2) {   // fln set by the previous substep's streams_rhs_finish_next
+#pragma unroll
+    for (int m = 0; m < NV; ++m) fl[fcell + fs_ * m] = acc[m];
   } else {
 #pragma unroll
     for (int m = 0; m < NV; ++m) fl[fcell + fs_ * m] = fl[fcell + fs_ * m] + acc[m];
@@ -438,7 +441,8 @@ int sweep_launch(const Box &b, SweepArgs A, hipStream_t st) {
 using namespace streams;
 
 // Fused sweep of direction dir (1..3): KEEP/WENO convective flux of faces 0..n plus the
-// viscous flux (visc=1) and the fl update of cells 1..n; init=1 also performs init_flux.
+// viscous flux (visc=1) and the fl update of cells 1..n; init=1 also performs init_flux,
+// init=2 overwrites fl without it (fln already set by streams_rhs_finish_next).
 extern "C" int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, int nz, int ng, int lmax_base,
                                  int nkeep, const double *rgas, const double *w_aux, const double *coeff_deriv1,
                                  const double *metric, const double *coord, double *fhat,
@@ -449,6 +453,7 @@ extern "C" int streams_rhs_sweep(int dir, int n_s, int nv_aux, int nx, int ny, i
                                  int flux_splitting, int visc, double r_univ, int init, double rhodt, double *fl,
                                  double *fln, void *stream) {
   if (dir < 1 || dir > 3) return STREAMS_E_DIMS;
+  if (init < 0 || init > 2) return STREAMS_E_OPTION;
   if (nv_aux < 2 * n_s + 13 || ng < lmax_base || ng < weno_scheme) return STREAMS_E_DIMS;
   if (lmax_base < 1 || lmax_base > LMX || weno_scheme < 1 || weno_scheme > 4) return STREAMS_E_OPTION;
   if (nsetcv < 1 || nsetcv > TMAXSET) return STREAMS_E_OPTION;

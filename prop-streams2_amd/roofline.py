@@ -13,9 +13,11 @@ its roofline fraction.  Counts per point, N_S species, NV = N_S + 4:
   compute_chemistry interior       read NV + 1, write N_S + 1 (Y, T), fl 2 N_S
   init/update_flux  interior       3 NV;   update_field 3 NV
   rhs_sweep         interior       read N_S + 6 (convective) + N_S + 3 (T, mu, k, rho D),
-                                   fl read+write 2 NV (+ NV for fln when init_flux is folded)
+                                   fl read+write 2 NV (+ NV for fln when init_flux is folded;
+                                   x with fln already set, "rhs_sweep_1_set": fl write NV)
   rhs_finish        interior       read 5 + N_S + 2 (u, v, w, mu, div/3, rho, T, Y),
                                    fl, fln, w read+write 6 NV
+  rhs_finish_next   interior       the same without the fl store: 5 NV
 """
 from __future__ import annotations
 
@@ -36,7 +38,9 @@ def per_point_doubles(name: str, ns: int) -> int:
         "rhs_sweep_1": (ns + 6) + (ns + 3) + 3 * nv,
         "rhs_sweep_2": (ns + 6) + (ns + 3) + 2 * nv,
         "rhs_sweep_3": (ns + 6) + (ns + 3) + 2 * nv,
+        "rhs_sweep_1_set": (ns + 6) + (ns + 3) + nv,
         "rhs_finish": 7 + ns + 6 * nv,
+        "rhs_finish_next": 7 + ns + 5 * nv,
     }
     return table.get(name, 0)
 

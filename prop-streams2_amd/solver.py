@@ -160,6 +160,10 @@ class Solver:
         self.fl = be.zeros((self.nv, nz, ny, nx))
         self.fln = be.zeros((self.nv, nz, ny, nx))
         self.fhat = be.zeros((self.nv, pz, py, px))
+        # fused substeps: fold each substep's init_flux into the previous substep's tail
+        # within rk_sync (bit-identical; saves one read and one write of fl and fln)
+        self.fold_substeps = True
+        self._fln_set = False
         # tables (Fortran layouts: store transposed C arrays)
         f = lambda a: be.asarray(np.asarray(a, dtype=np.float64).T)  # noqa: E731
         self.t_rgas = be.asarray(tables.rgas)
@@ -441,6 +445,9 @@ class Solver:
         """One substep of rk_sync (multideal_gpu.F90:562-618); simpler: rk_sync with
         simpler_splitting = 1, whose stages accumulate fl - fl_sav."""
         n = self.num
+        if self._fln_set:
+            raise RuntimeError("the previous fused substep already did this substep's init_flux: "
+                               "continue with rk_substep_fused")
         rhodt = WRAY_RHO[istep] * self.dt
         gamdt = WRAY_GAM[istep] * self.dt
         be = self.be
@@ -519,11 +526,15 @@ class Solver:
                 and self.num.enable_les == 0
                 and self.be.has("rhs_sweep") and self.be.has("rhs_finish"))
 
-    def rk_substep_fused(self, istep: int):
+    def rk_substep_fused(self, istep: int, fold_next: bool = False):
         """One RK substep in five passes: halo + aux, sensor, fused x/y/z sweeps (with
         init_flux folded into the first), and the pointwise tail (reduced viscous terms,
-        chemistry, update_flux, update_field), then the physical boundaries."""
+        chemistry, update_flux, update_field), then the physical boundaries.  fold_next (not
+        for the last substep): the tail also does substep istep + 1's init_flux
+        (streams_rhs_finish_next), so fl is left unset and that substep's first sweep skips
+        reading it."""
         n, tb, be = self.num, self.tb, self.be
+        fold_next = fold_next and istep < 2 and be.has("rhs_finish_next")
         rhodt = WRAY_RHO[istep] * self.dt
         gamdt = WRAY_GAM[istep] * self.dt
         self.bcswap()
@@ -541,16 +552,21 @@ class Solver:
                     self.t_rgas, self.w_aux, self.cd1, self.dxi[d], coord, self.fhat, self.force_zero_flux[2 * d],
                     self.force_zero_flux[2 * d + 1], n.weno_scheme, n.weno_version, n.sensor_threshold, self.t_cv,
                     self.t_cp, tb.nsetcv, self.t_trange, tb.indx_cp_l, tb.indx_cp_r, self.t_epo, TOL_ITER_NR,
-                    self.rho0, self.u0, n.flux_splitting, int(nd > 1), R_UNIV, int(first), rhodt, self.fl, self.fln,
-                    be.stream)
+                    self.rho0, self.u0, n.flux_splitting, int(nd > 1), R_UNIV,
+                    (2 if self._fln_set else 1) if first else 0, rhodt, self.fl, self.fln, be.stream)
             first = False
         chem = int(n.enable_chemistry == 1 and self.nr > 0)
         if chem and getattr(self, "t_kinpack", None) is None:
             self.t_kinpack = be.asarray(capi.kinetics_pack(tb, be.lib))
         kin = (self.t_mw, self.t_kinpack, self.t_kc) if chem else (self.t_mw, None, None)
-        be.call("rhs_finish", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, n.visc_order, self.w,
-                self.w_aux, self.fl, self.fln, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], 1, chem, self.nr,
-                *kin, tb.num_t_tab, tb.t_min_tab, tb.dt_tab, R_UNIV, gamdt, None, be.stream)
+        args = (self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, n.visc_order, self.w, self.w_aux, self.fl,
+                self.fln, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], 1, chem, self.nr, *kin, tb.num_t_tab,
+                tb.t_min_tab, tb.dt_tab, R_UNIV, gamdt)
+        if fold_next:
+            be.call("rhs_finish_next", *args, WRAY_RHO[istep + 1] * self.dt, None, be.stream)
+        else:
+            be.call("rhs_finish", *args, None, be.stream)
+        self._fln_set = fold_next
         self.update_ghost()
 
     def rk_sync(self, fused: Optional[bool] = None, simpler: bool = False):
@@ -560,7 +576,7 @@ class Solver:
             fused = self.fused_supported()
         for istep in range(3):
             if fused and not simpler:
-                self.rk_substep_fused(istep)
+                self.rk_substep_fused(istep, fold_next=self.fold_substeps)
             else:
                 self.rk_substep(istep, simpler=simpler)
 

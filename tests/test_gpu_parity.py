@@ -501,6 +501,35 @@ def test_fused_substep_tube_like(pkg, h2_tables, hip_backend, cpu_backend):
     assert err_w.max() < 1e-13, err_w
 
 
+def test_folded_substeps_are_bit_identical(pkg, h2_tables, hip_backend):
+    """rk_sync folds each substep's init_flux into the previous substep's tail
+    (streams_rhs_finish_next, then an x sweep with init=2).  Two RK steps give the same bits
+    in w, fl and fln as the unfolded chain (streams_rhs_finish, then init=1), with explicit
+    chemistry, and again with walls and WENO faces."""
+    assert hip_backend.has("rhs_finish_next")
+    cases = ((None, (0,) * 6),
+             (pkg.solver.Numerics(sensor_threshold=0.05, sensor_type=0, nkeep=1), (8, 2, 0, 0, 2, 2)))
+    for numerics, bctags in cases:
+        w, tg, info = box_state(pkg, h2_tables, SHAPE, seed=31, temperature=1500.0, perturb_y=0.3)
+        out = []
+        dt = None
+        for fold in (True, False):
+            s = pkg.solver.Solver(hip_backend, h2_tables, *SHAPE, NG, [info["length"]] * 3, numerics=numerics,
+                                  bctags=bctags, rho0=info["rho0"], u0=info["u_rms"])
+            s.fold_substeps = fold
+            s.set_state(w, tg)
+            s.compute_aux()
+            dt = s.compute_dt(0.5) if dt is None else dt
+            s.dt = dt
+            print(f"[fold] bctags {bctags} fold {fold}", flush=True)
+            for _ in range(2):
+                s.rk_sync()
+            s.be.sync()
+            out.append([s.be.to_numpy(a).copy() for a in (s.w, s.fl, s.fln)])
+        for a, b in zip(*out):
+            assert np.array_equal(a, b)
+
+
 def test_fused_substep_syngas_mixing_layer(pkg, hip_backend, cpu_backend):
     """The 14-species Keromnes-2013 syngas mixing layer (the C4 configuration at 24 x 16 x 12)
     with the sensor threshold set so that about 10 % of the faces take the WENO flux: one
