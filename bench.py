@@ -182,8 +182,10 @@ def reactor_bench(pkg, n: int, n_fp64: int = 262144, reps: int = 1):
                        "p log-uniform 0.5-20 atm, Dirichlet(1) mass fractions, dt = 1e-6 s"
                        if n == 1 << 21 else f"{n} reactors of the batched-chemistry workload"}
     finals = {}
-    # the persistent grid's workgroups per CU, tuned on a 2^16-reactor sample of the workload
-    ns_ = min(n, 1 << 16)
+    # the persistent grid's workgroups per CU, tuned on a 2^17-reactor sample of the workload
+    # (larger than the GPU holds at one wave per SIMD, so it runs the instance the full batch
+    # runs)
+    ns_ = min(n, 1 << 17)
     G.tune(be.asarray(np.ascontiguousarray(w[:, :ns_])), be.asarray(tg[:ns_]), be.asarray(dttry[:ns_]),
            be.asarray(tend[:ns_]), mixed=1)
     out["grid_tuning"] = G.tuning

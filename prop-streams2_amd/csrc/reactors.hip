@@ -349,8 +349,28 @@ __device__ inline void integrate_one(const Rhs &f, const ReactorArgs &A, unsigne
   }
 }
 
-template <int NS, int LO, int MODE, bool CP>
-__global__ void __launch_bounds__(256) reactor_kernel(ReactorArgs A) {
+// Waves per SIMD the compiler budgets registers for.  Every form spills (the per-lane
+// Jacobian, its factors and the four stage vectors do not fit).  A batch larger than the
+// GPU holds at one wave per SIMD (CUs x 256 lanes) runs the WIDE instance: more resident
+// waves then hide the spill and table latencies better than fewer, larger register files.
+// Measured on the 2^19-reactor H2-air workload (mixed / FP64 species block, reactor-solves/s
+// at 1, 2, 3, 4 waves): 155k/171k, 237k/257k, 253k/283k, 221k/234k; on the 14-species
+// mixing-layer cells best at 2 waves in both forms.  A smaller batch (the serving loop's)
+// cannot fill the extra waves, and each lane is faster at one wave: the reactor server ran
+// 15 % slower with the wide instances.  The plain FP64 form and the mixed form beyond 9
+// species keep one wave in both instances: with the narrow instance beside them the
+// compiler does not meet a wider budget for them (their shared code keeps the one-wave
+// register file), so a wide target would only add spills.
+template <int NS, int MODE, bool WIDE>
+constexpr int reactor_waves() {
+  return (!WIDE || MODE == 0 || (MODE == 1 && NS > 9)) ? 1 : (NS <= 9 ? 3 : 2);
+}
+
+// flatten: the integrator and right-hand sides are inlined into each instance, so the wide
+// and narrow kernels do not share callees (a shared callee gets one register budget for both)
+template <int NS, int LO, int MODE, bool CP, bool WIDE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(reactor_waves<NS, MODE, WIDE>()), flatten))
+reactor_kernel(ReactorArgs A) {
   constexpr int WHICH = CP ? (TH_CP | TH_CPH) : (TH_CV | TH_CVH);
   // the plain FP64 form reads the NASA tables from global memory in the reference's forms
   __shared__ double tlds[MODE == 0 ? 1 : thermo_lds_doubles<NS>(WHICH)];
@@ -391,14 +411,19 @@ __global__ void zero_counter_kernel(unsigned *c) { *c = 0u; }
 // behind the resident ones start after it has run out and exit at once.  The occupancy
 // API is not used: its query has returned error 999 after many other kernels had been
 // launched in the process.
-inline int reactor_grid(int wg_per_cu, long nbatch) {
+inline int device_cus() {
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       ncu <= 0) {
     (void)hipGetLastError();
     ncu = 256;
   }
-  const long per = wg_per_cu > 0 ? wg_per_cu : 4;
+  return ncu;
+}
+
+inline int reactor_grid(int ncu, int wg_per_cu, int min_per_cu, long nbatch) {
+  long per = wg_per_cu > 0 ? wg_per_cu : 4;
+  per = per < min_per_cu ? min_per_cu : per;
   const long need = (nbatch + 255) / 256;
   const long g = ncu * per;
   return (int)(g < need ? g : need);
@@ -406,9 +431,16 @@ inline int reactor_grid(int wg_per_cu, long nbatch) {
 
 template <int NS, int LO, int MODE, bool CP>
 int reactor_launch_t(const ReactorArgs &A, int wg_per_cu, hipStream_t st) {
-  const int g = reactor_grid(wg_per_cu, A.nbatch);
-  if (g <= 0) return STREAMS_E_DIMS;
-  reactor_kernel<NS, LO, MODE, CP><<<g, 256, 0, st>>>(A);
+  const int ncu = device_cus();
+  if (A.nbatch > (long)ncu * 256) {
+    // a workgroup is one wave per SIMD: the wide instance needs as many per CU as its waves
+    const int g = reactor_grid(ncu, wg_per_cu, reactor_waves<NS, MODE, true>(), A.nbatch);
+    reactor_kernel<NS, LO, MODE, CP, true><<<g, 256, 0, st>>>(A);
+  } else {
+    const int g = reactor_grid(ncu, wg_per_cu, 1, A.nbatch);
+    if (g <= 0) return STREAMS_E_DIMS;
+    reactor_kernel<NS, LO, MODE, CP, false><<<g, 256, 0, st>>>(A);
+  }
   return hip_status();
 }
 

@@ -300,10 +300,11 @@ def test_mixed_form_on_the_batched_workload(pkg, hip):
 
 def test_grid_size_does_not_change_results(pkg, hip):
     """The persistent grid only decides which lane takes which reactor from the shared
-    counter.  600,000 reactors need 2,344 workgroups of 256 lanes, so 1 workgroup per CU
-    (256 workgroups, all resident) and 8 per CU (2,048 workgroups, more than the GPU holds
-    at once for this register-heavy kernel: the queued ones start after the counter has run
-    out and exit at once) give bit-identical states and step counts."""
+    counter.  600,000 reactors need 2,344 workgroups of 256 lanes and run the wide (2 waves
+    per SIMD) instance, so 1 workgroup per CU (raised to that instance's 2: 512 workgroups,
+    all resident) and 8 per CU (2,048 workgroups, more than the GPU holds at once for this
+    register-heavy kernel: the queued ones start after the counter has run out and exit at
+    once) give bit-identical states and step counts."""
     tables = pkg.cases.h2_air_mechanism().tables()
     n = 600000
     w, tg, _ = pkg.cases.reactor_batch(tables, n, seed=5)
@@ -321,4 +322,28 @@ def test_grid_size_does_not_change_results(pkg, hip):
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
     np.testing.assert_array_equal(out[0][2], out[1][2])
+    assert out[0][1].sum() > 0
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_narrow_and_wide_instances_agree(pkg, hip, mixed):
+    """A batch that fits the GPU at one wave per SIMD runs the narrow instance of the reactor
+    kernel, a larger one the wide instance compiled for more waves (fewer registers, more
+    spills).  The arithmetic is the same: the first 2,000 reactors of a 70,000-reactor
+    batch (wide) end bit for bit where the same 2,000 end alone (narrow)."""
+    tables = pkg.cases.h2_air_mechanism().tables()
+    n, m = 70000, 2000
+    w, tg, _ = pkg.cases.reactor_batch(tables, n, seed=9)
+    G = pkg.reactors.ReactorBatchGPU(hip, tables, tol=1e-8)
+    out = []
+    for nb in (m, n):
+        print(f"[reactors] narrow/wide, mixed={mixed}, batch {nb}", flush=True)
+        wd, td = hip.asarray(np.ascontiguousarray(w[:, :nb])), hip.asarray(tg[:nb])
+        na = hip.zeros((nb,), "i4")
+        G.integrate(wd, td, hip.asarray(np.full(nb, pkg.reactors.ROS_DTTRY)), hip.asarray(np.full(nb, 2e-7)), na,
+                    mixed=mixed)
+        hip.sync()
+        out.append((hip.to_numpy(wd)[:, :m], hip.to_numpy(na)[:m]))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
     assert out[0][1].sum() > 0
