@@ -54,7 +54,9 @@ int streams_eval_aux(int n_s, int nx, int ny, int nz, int ng, int nv, int nv_aux
                      int num_t_tab, int n_eoi, const double *aw_eoi, const int *nainsp,
                      const double *beta0, const double *coeff_eoi);
 
-/* eval_aux2_cuf, kernels_gpu.F90:6310-6382: Ducros sensor and div/3 on the interior. */
+/* eval_aux2_cuf, kernels_gpu.F90:6310-6382: Ducros sensor and div/3 on the interior.
+ * sensor_type < 0 (no counterpart in the reference) computes div/3 only and leaves the
+ * Ducros column untouched, for substeps whose sensor threshold is at least 1. */
 int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv_aux, int visc_order,
                       double *w_aux, const double *coeff_deriv1, const double *dcsidx,
                       const double *detady, const double *dzitdz, double eps_sensor,

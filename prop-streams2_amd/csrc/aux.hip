@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(256) eval_transport_kernel(Box b, int istart, 
   }
 }
 
-template <int NS, int LB>
+// DUC = false: div/3 only (the sensor is not wanted), from the three diagonal derivatives
+template <int NS, int LB, bool DUC>
 __global__ void __launch_bounds__(256) eval_aux2_kernel(Box b, int lmax, double *__restrict__ w_aux,
                                                         const double *__restrict__ cd1,
                                                         const double *__restrict__ dcsidx,
@@ -258,23 +259,27 @@ __global__ void __launch_bounds__(256) eval_aux2_kernel(Box b, int lmax, double 
     long q = pt;
     opaque(q);
     ux = ux + ccl * (U[q + l * sx] - U[q - l * sx]);
-    vx = vx + ccl * (V[q + l * sx] - V[q - l * sx]);
-    wx = wx + ccl * (Wv[q + l * sx] - Wv[q - l * sx]);
+    if (DUC) {
+      vx = vx + ccl * (V[q + l * sx] - V[q - l * sx]);
+      wx = wx + ccl * (Wv[q + l * sx] - Wv[q - l * sx]);
+    }
     __builtin_amdgcn_sched_barrier(0);
     q = pt + l * sy;
     opaque(q);
     long qm = pt - l * sy;
     opaque(qm);
-    uy = uy + ccl * (U[q] - U[qm]);
+    if (DUC) uy = uy + ccl * (U[q] - U[qm]);
     vy = vy + ccl * (V[q] - V[qm]);
-    wy = wy + ccl * (Wv[q] - Wv[qm]);
+    if (DUC) wy = wy + ccl * (Wv[q] - Wv[qm]);
     __builtin_amdgcn_sched_barrier(0);
     q = pt + l * sz;
     opaque(q);
     qm = pt - l * sz;
     opaque(qm);
-    uz = uz + ccl * (U[q] - U[qm]);
-    vz = vz + ccl * (V[q] - V[qm]);
+    if (DUC) {
+      uz = uz + ccl * (U[q] - U[qm]);
+      vz = vz + ccl * (V[q] - V[qm]);
+    }
     wz = wz + ccl * (Wv[q] - Wv[qm]);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -283,11 +288,13 @@ __global__ void __launch_bounds__(256) eval_aux2_kernel(Box b, int lmax, double 
   uy *= dy; vy *= dy; wy *= dy;
   uz *= dz; vz *= dz; wz *= dz;
   const double div = ux + vy + wz;
-  const double ox = wy - vz, oy = uz - wx, oz = vx - uy;
-  const double omod2 = ox * ox + oy * oy + oz * oz;
-  double s = -div / sqrt(omod2 + div * div + eps_sensor);
-  if (sensor_type != 0 && s < 0.0) s = 0.0;
-  w_aux[pt + ps * (X::J_DUC - 1)] = s * s;
+  if (DUC) {
+    const double ox = wy - vz, oy = uz - wx, oz = vx - uy;
+    const double omod2 = ox * ox + oy * oy + oz * oz;
+    double s = -div / sqrt(omod2 + div * div + eps_sensor);
+    if (sensor_type != 0 && s < 0.0) s = 0.0;
+    w_aux[pt + ps * (X::J_DUC - 1)] = s * s;
+  }
   w_aux[pt + ps * (X::J_DIV - 1)] = div / 3.0;
 }
 
@@ -339,12 +346,22 @@ extern "C" int streams_eval_aux2(int n_s, int nx, int ny, int nz, int ng, int nv
   const StencilRows SR = stencil_rows(b);
   if (SR.tiles(nz) > 0xffffff00u) return STREAMS_E_DIMS;
   const unsigned gm = xcd_grid(SR.tiles(nz));
-  STREAMS_DISPATCH_NS(n_s, (lmax == 3 ? eval_aux2_kernel<NS, 3><<<gm, 256, 0, as_stream(stream)>>>(
-                                            b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
-                                            sensor_type, SR)
-                                      : eval_aux2_kernel<NS, 4><<<gm, 256, 0, as_stream(stream)>>>(
-                                            b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
-                                            sensor_type, SR)));
+  hipStream_t st = as_stream(stream);
+  if (sensor_type >= 0) {
+    STREAMS_DISPATCH_NS(n_s, (lmax == 3 ? eval_aux2_kernel<NS, 3, true><<<gm, 256, 0, st>>>(
+                                              b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                              sensor_type, SR)
+                                        : eval_aux2_kernel<NS, 4, true><<<gm, 256, 0, st>>>(
+                                              b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                              sensor_type, SR)));
+  } else {
+    STREAMS_DISPATCH_NS(n_s, (lmax == 3 ? eval_aux2_kernel<NS, 3, false><<<gm, 256, 0, st>>>(
+                                              b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                              sensor_type, SR)
+                                        : eval_aux2_kernel<NS, 4, false><<<gm, 256, 0, st>>>(
+                                              b, lmax, w_aux, coeff_deriv1, dcsidx, detady, dzitdz, eps_sensor,
+                                              sensor_type, SR)));
+  }
   return hip_status();
 }
 

@@ -264,10 +264,12 @@ class Solver:
                      self.t_mwinv, self.t_visc, self.t_lambda, self.t_diff, tb.num_t_tab, 1, self.t_eoi,
                      self.t_nainsp, self.t_beta0, self.t_eoi)
 
-    def eval_aux2(self):
+    def eval_aux2(self, ducros: bool = True):
+        """eval_aux2_cuf; ducros=False (the fused substeps with the sensor off) computes
+        div/3 only and leaves the Ducros column as it was."""
         self.be.call("eval_aux2", self.ns, self.nx, self.ny, self.nz, self.ng, self.nv_aux, self.num.visc_order,
                      self.w_aux, self.cd1, self.dxi[0], self.dxi[1], self.dxi[2], self.eps_sensor,
-                     self.num.sensor_type, self.be.stream)
+                     self.num.sensor_type if ducros else -1, self.be.stream)
 
     def compute_aux_les(self):
         """compute_aux_les (multideal_gpu.F90:445-498): Y, rho and velocity, the WALE eddy
@@ -302,9 +304,10 @@ class Solver:
                      self.dxi[0], self.dxi[1], self.dxi[2], self.t_h298, n.enable_pasr, n.les_c_yoshi, n.les_c_mix,
                      n.les_c_eps, self.be.stream)
 
-    def compute_aux(self, central: int = 1, ghost: int = 1):
+    def compute_aux(self, central: int = 1, ghost: int = 1, ducros: bool = True):
         """compute_aux (multideal_gpu.F90:361-443); with LES every call site of the
-        reference calls compute_aux_les instead, whatever the box."""
+        reference calls compute_aux_les instead, whatever the box.  ducros=False: see
+        eval_aux2."""
         if self.num.enable_les > 0:
             self.compute_aux_les()
             return
@@ -319,7 +322,7 @@ class Solver:
                         (1 - ng, nx + ng, 1 - ng, ny + ng, 1 - ng, 0), (1 - ng, nx + ng, 1 - ng, ny + ng, nz + 1, nz + ng)):
                 self.eval_aux(box)
         if ghost == 1:
-            self.eval_aux2()
+            self.eval_aux2(ducros)
 
     def bcswap(self):
         self.be.call("bcswap", self.nx, self.ny, self.nz, self.ng, self.nv, self.w, self.t_periodic, self.be.stream)
@@ -538,10 +541,14 @@ class Solver:
         rhodt = WRAY_RHO[istep] * self.dt
         gamdt = WRAY_GAM[istep] * self.dt
         self.bcswap()
-        self.compute_aux()
+        # the Ducros sensor is read only by the sweeps' shock test, which a threshold of 1 or
+        # more (the sensor's upper bound) switches off
+        sensor = n.sensor_threshold < 1.0
+        self.compute_aux(ducros=sensor)
         jduc, jdiv = self.ns + 9, self.ns + 10
         be.call("bcextr_var", self.nx, self.ny, self.nz, self.ng, self.w_aux[jdiv - 1], be.stream)
-        self.bcswap_var(jduc)
+        if sensor:
+            self.bcswap_var(jduc)
         self.bcswap_var(jdiv)
         lmax = n.ep_order // 2
         first = True

@@ -102,6 +102,24 @@ def test_eval_aux2(pkg, h2_tables, hip_backend, cpu_backend, sensor_type):
     assert err.max() < TOL, err
 
 
+def test_eval_aux2_div_only(pkg, h2_tables, hip_backend):
+    """sensor_type = -1 (the fused substeps with the sensor off): div/3 with the same bits as
+    the full eval_aux2, and the Ducros column left as it was."""
+    g, _, _ = make_pair(pkg, h2_tables, hip_backend, hip_backend)
+    g.eval_aux()
+    ns = h2_tables.n_species
+    g.eval_aux2()
+    hip_backend.sync()
+    full = hip_backend.to_numpy(g.w_aux).copy()
+    g.w_aux[ns + 8].fill_(-7.0)
+    g.w_aux[ns + 9].fill_(0.0)
+    g.eval_aux2(ducros=False)
+    hip_backend.sync()
+    part = hip_backend.to_numpy(g.w_aux)
+    assert np.array_equal(interior(part, NG)[ns + 9], interior(full, NG)[ns + 9])
+    assert (part[ns + 8] == -7.0).all()
+
+
 @pytest.mark.parametrize("periodic", [(1, 1, 1), (1, 0, 1), (0, 1, 0)])
 def test_bcswap_bit_exact(pkg, hip_backend, cpu_backend, periodic):
     rng = np.random.default_rng(5)
