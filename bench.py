@@ -70,6 +70,8 @@ class TimedBackend:
             tag = key + ("_set" if args[-5] == 2 else "")
         elif name == "rhs_finish_next":
             key, tag = "rhs_finish", name
+        elif name == "eval_aux2":
+            key, tag = name, ("eval_aux2_div" if args[-2] < 0 else name)     # sensor_type < 0: div/3 only
         else:
             key = tag = name
         if key not in self.names:

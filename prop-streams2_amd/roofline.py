@@ -6,7 +6,8 @@ that keeps its reuse on chip moves, so ``bytes / duration`` against the 8 TB/s H
 its roofline fraction.  Counts per point, N_S species, NV = N_S + 4:
 
   eval_aux          padded points  read NV + 1 (w, T guess), write 2 N_S + 11
-  eval_aux2         interior       read 3 (u, v, w), write 2 (Ducros, div/3)
+  eval_aux2         interior       read 3 (u, v, w), write 2 (Ducros, div/3); without the
+                                   sensor ("eval_aux2_div") write 1
   euler_{x,y,z}     interior       read N_S + 6 (Y, rho, u, v, w, h, p), fl read+write 2 NV
   visflx_{x,y,z}    interior       read 2 N_S + 6 (u, v, w, T, mu, k, Y, rho D), fl 2 NV
   visflx_reduced    interior       read 5 (u, v, w, mu, div/3), fl 2 x 4
@@ -29,7 +30,7 @@ def per_point_doubles(name: str, ns: int) -> int:
     nv = ns + 4
     table = {
         "eval_aux": nv + 1 + 2 * ns + 11,
-        "eval_aux2": 5,
+        "eval_aux2": 5, "eval_aux2_div": 4,
         "euler_x": ns + 6 + 2 * nv, "euler_y": ns + 6 + 2 * nv, "euler_z": ns + 6 + 2 * nv,
         "visflx_x": 2 * ns + 6 + 2 * nv, "visflx_y": 2 * ns + 6 + 2 * nv, "visflx_z": 2 * ns + 6 + 2 * nv,
         "visflx_reduced": 5 + 8,
